@@ -1,0 +1,173 @@
+#!/usr/bin/env python
+"""Throughput of the fused RIME forward on one MI355X.
+
+Workload (default): 64 antennas (2016 baselines), 100 times, 64 channels at
+1.40-1.42 GHz, 1000 point + 500 gaussian + 100 sersic sources, beam cube
+65x65x32, synthetic inputs (oracle/synth.py, seed 1001). A step is one
+fused forward (model visibilities, chi^2 terms and X2) over the whole
+problem with every input resident in HBM. Metric: source-visibility terms
+per second (one term = one source, time, baseline and channel with all four
+correlations).
+
+Both precisions are timed; ``value`` is the float32 run (float64 phase
+argument and beam coordinates), the float64 run is reported beside it.
+The roofline object is for the dominant kernel (the fused forward kernel):
+algorithmic FLOPs per launch over its HIP-event-timed launch duration,
+against the MI355X vector peak. cpu_baseline times the numpy restatement of
+the reference's CPU chain on a bounded slice of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=10)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--workload', default='forward_64ant')
+    p.add_argument('--dtypes', default='f32,f64')
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    return p.parse_args()
+
+
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or 'unknown'
+
+
+def time_forward(w, dtype, steps, warmup):
+    import numpy as np
+    import torch
+    from montblanc_amd import rime, roofline
+    from oracle import synth
+
+    counts = (w['npsrc'], w['ngsrc'], w['nssrc'])
+    a = synth.make_inputs(**w, seed=1001)
+    arrays = rime.to_device(synth.as_dtype(a, np.float32 if dtype == 'f32' else np.float64))
+    del a
+    out = rime.alloc_outputs(arrays)
+    # observed = model + CN(0, 0.1), the model from this GPU path (untimed)
+    rime.forward(arrays, counts, outputs=out)
+    g = torch.Generator(device='cuda').manual_seed(7)
+    noise = torch.randn(out['model_vis'].shape + (2,), generator=g, device='cuda',
+                        dtype=arrays['uvw'].dtype) * 0.1
+    arrays['observed_vis'] = (out['model_vis'] + torch.view_as_complex(noise)).contiguous()
+    del noise
+    for _ in range(warmup):
+        rime.forward(arrays, counts, outputs=out)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        rime.forward(arrays, counts, outputs=out)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
+    x2 = float(out['X2'].item())
+    nbl = w['na'] * (w['na'] - 1) // 2
+    n_terms = roofline.terms(w['na'], w['ntime'], w['nchan'], sum(counts), nbl)
+    flops = roofline.forward_flops(w['na'], w['ntime'], w['nchan'], *counts, nbl=nbl)
+    ms_mean = sum(ev) / len(ev)
+    ms_median = ev[len(ev) // 2]
+    del arrays, out
+    torch.cuda.empty_cache()
+    return dict(ms_per_step=wall * 1e3 / steps, ms_event_mean=ms_mean, ms_event_median=ms_median,
+                terms=n_terms, flops=flops, X2=x2,
+                terms_per_s=n_terms / (wall / steps),
+                tflops=flops / (ms_mean * 1e-3) / 1e12)
+
+
+def cpu_baseline(w):
+    """numpy restatement of the reference CPU chain on one time step and the
+    first 16 channels, all sources and baselines (~10-30 s single core)."""
+    import numpy as np
+    from oracle import rime_np, synth
+    sample = dict(w, ntime=1, nchan=16)
+    a = synth.make_inputs(**sample, seed=1001)
+    t0 = time.perf_counter()
+    rime_np.rime(a, w['npsrc'], w['ngsrc'], w['nssrc'], src_chunk=64)
+    dt = time.perf_counter() - t0
+    nbl = w['na'] * (w['na'] - 1) // 2
+    n = (w['npsrc'] + w['ngsrc'] + w['nssrc']) * 1 * nbl * 16
+    return dict(value=n / dt, unit='source-vis terms/s', cores=1, kind='port',
+                sample='%s with ntime=1, nchan=16 (all %d sources, %d baselines): %.3g terms '
+                       'in %.1f s, numpy float64, one process; host %s, os.cpu_count()=%d'
+                       % (w.get('name', 'workload'), w['npsrc'] + w['ngsrc'] + w['nssrc'], nbl,
+                          n, dt, cpu_model(), os.cpu_count() or 0))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU only (the hot path has no '
+              'multi-GPU collectives); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from montblanc_amd import roofline
+    from oracle import synth
+
+    w = dict(synth.WORKLOADS[args.workload])
+    w['name'] = args.workload
+    wl = {k: v for k, v in w.items() if k != 'name'}
+    results = {}
+    for dt in args.dtypes.split(','):
+        results[dt] = time_forward(wl, dt, args.steps, args.warmup)
+        print('# %s: %.3f ms/step (events mean %.3f), %.4g terms/s, %.2f TFLOP/s'
+              % (dt, results[dt]['ms_per_step'], results[dt]['ms_event_mean'],
+                 results[dt]['terms_per_s'], results[dt]['tflops']), file=sys.stderr, flush=True)
+    primary = 'f32' if 'f32' in results else next(iter(results))
+    r = results[primary]
+    peak = roofline.PEAK_TFLOPS[primary]
+    line = {
+        'metric': 'source-vis terms/s, fused RIME forward (model vis + chi^2), 1xMI355X',
+        'value': r['terms_per_s'],
+        'unit': 'source-vis terms/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': r['ms_per_step'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': primary,
+        'data': 'synthetic (oracle/synth.py seed 1001: uvw U(+-3000 m), lm U(+-0.01), '
+                'lognormal Stokes, random beam cube)',
+        'config': {'workload': args.workload, 'na': w['na'], 'nbl': w['na'] * (w['na'] - 1) // 2,
+                   'ntime': w['ntime'], 'nchan': w['nchan'], 'npsrc': w['npsrc'],
+                   'ngsrc': w['ngsrc'], 'nssrc': w['nssrc'], 'beam': list(w['beam']),
+                   'phase_argument': 'f64'},
+        'roofline': {'bound': 'valu', 'achieved': r['tflops'], 'peak': peak,
+                     'unit': 'TFLOP/s', 'frac': r['tflops'] / peak, 'traffic': None,
+                     'kernel': 'k_forward', 'flops_per_launch': r['flops']},
+        'per_dtype': {k: {'terms_per_s': v['terms_per_s'], 'ms_per_step': v['ms_per_step'],
+                          'ms_event_mean': v['ms_event_mean'], 'tflops': v['tflops'],
+                          'frac_of_peak': v['tflops'] / roofline.PEAK_TFLOPS[k], 'X2': v['X2']}
+                      for k, v in results.items()},
+    }
+    if not args.no_cpu_baseline:
+        line['cpu_baseline'] = cpu_baseline(w)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

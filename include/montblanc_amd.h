@@ -1,0 +1,190 @@
+/*
+ * montblanc_amd.h -- C-ABI of the MI355X (gfx950) v4 RIME hot path.
+ *
+ * Every entry point is stateless: dimensions come by pointer to an
+ * mbx_dims (the counterpart of the reference's __constant__
+ * rime_const_data struct, montblanc/util/const_data.py:39-169), arrays are
+ * caller-owned DEVICE pointers in the reference's v4 layouts
+ * (montblanc/impl/rime/v4/config.py:131-247), complex values are
+ * interleaved (re, im), and work is enqueued on the given hipStream_t
+ * (passed as void* so the header needs no HIP include).
+ *
+ * Return codes: 0 = success, < 0 = argument validation failure
+ * (MBX_E_*), > 0 = a hipError_t from the launch.
+ * mbx_last_error() returns a static string describing the last failure on
+ * the calling thread.
+ *
+ * dtype suffix: _f32 = float/complex64 arrays (phase argument and beam
+ * grid coordinates evaluated in float64), _f64 = double/complex128.
+ */
+#ifndef MONTBLANC_AMD_H
+#define MONTBLANC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBX_ABI_VERSION 1
+
+enum {
+    MBX_OK = 0,
+    MBX_E_NULL = -1,        /* required pointer is NULL              */
+    MBX_E_DIMS = -2,        /* inconsistent or zero dimensions       */
+    MBX_E_EXTENT = -3,      /* lower/upper extents outside the sizes */
+    MBX_E_WORKSPACE = -4,   /* workspace too small                   */
+    MBX_E_UNSUPPORTED = -5, /* npol != 4 or an unsupported option    */
+    MBX_E_NO_DEVICE = -6    /* no HIP device visible                 */
+};
+
+/* One dimension: montblanc/util/const_data.py:51-86 (dim_field). */
+typedef struct mbx_dim {
+    uint32_t global_size;
+    uint32_t local_size;
+    uint32_t lower_extent;
+    uint32_t upper_extent;
+} mbx_dim;
+
+/* All dimensions the path reads (const_data.py:118-145; the nsrc extents
+ * are the sums of the per-type extents, as RimeConstStruct.update sets). */
+typedef struct mbx_dims {
+    mbx_dim ntime, na, nbl, nchan, npol, npolchan;
+    mbx_dim nsrc, npsrc, ngsrc, nssrc;
+    mbx_dim beam_lw, beam_mh, beam_nud;
+} mbx_dims;
+
+/* Beam cube extents: solver properties beam_{ll,lm,lfreq,ul,um,ufreq}
+ * (montblanc/impl/rime/v4/config.py:62-68). */
+typedef struct mbx_beam_extents {
+    double ll, lm, lfreq, ul, um, ufreq;
+} mbx_beam_extents;
+
+/* Options of the coherency sum: the stamped kernel variants
+ * apply_weights x vis_output (RimeSumCoherencies.py:405-464). */
+enum {
+    MBX_APPLY_WEIGHTS = 1,    /* weight_vector multiplies chi^2 terms      */
+    MBX_RESIDUAL_OUTPUT = 2   /* write obs - model instead of the model     */
+};
+
+/* Constants the kernels take from solver properties
+ * (v4/config.py:55-57). */
+typedef struct mbx_consts {
+    double gauss_scale;     /* fwhm2int*sqrt(2)*pi/C */
+    double two_pi_over_c;   /* 2*pi/C                */
+    double sigma_sqrd;      /* chi^2 divisor when weights are off */
+} mbx_consts;
+
+int mbx_abi_version(void);
+const char *mbx_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int mbx_device_count(void);
+
+/* ---- Stage entry points (one per v4 kernel node) -------------------- */
+
+/* B_sqrt[s,t,c,:] = sqrt(B(stokes[s,t], alpha[s,t]) * (f/rf)^alpha).
+ * Replaces rime_jones_B_sqrt_{float,double}
+ * (montblanc/impl/rime/v4/gpu/RimeBSqrt.py:130-148). */
+int mbx_b_sqrt_f32(const mbx_dims *d, const float *stokes, const float *alpha,
+                   const float *frequency, const float *ref_frequency,
+                   void *B_sqrt, void *stream);
+int mbx_b_sqrt_f64(const mbx_dims *d, const double *stokes, const double *alpha,
+                   const double *frequency, const double *ref_frequency,
+                   void *B_sqrt, void *stream);
+
+/* jones[s,t,a,c,:] = E (trilinear beam-cube lookup, amplitude-normalised).
+ * Replaces rime_jones_E_beam_{float,double} (RimeEBeam.py:348-366). */
+int mbx_e_beam_f32(const mbx_dims *d, const float *lm, const float *parallactic_angles,
+                   const float *point_errors, const float *antenna_scaling,
+                   const float *frequency, const void *E_beam, void *jones,
+                   const mbx_beam_extents *beam, void *stream);
+int mbx_e_beam_f64(const mbx_dims *d, const double *lm, const double *parallactic_angles,
+                   const double *point_errors, const double *antenna_scaling,
+                   const double *frequency, const void *E_beam, void *jones,
+                   const mbx_beam_extents *beam, void *stream);
+
+/* jones <- jones . (K * B_sqrt), K = exp(-2 pi i f/C (ul + vm + w(n-1))).
+ * Replaces rime_jones_EKBSqrt_{float,double} (RimeEKBSqrt.py:182-196). */
+int mbx_ekb_sqrt_f32(const mbx_dims *d, const float *uvw, const float *lm,
+                     const float *frequency, const void *B_sqrt, void *jones,
+                     const mbx_consts *k, void *stream);
+int mbx_ekb_sqrt_f64(const mbx_dims *d, const double *uvw, const double *lm,
+                     const double *frequency, const void *B_sqrt, void *jones,
+                     const mbx_consts *k, void *stream);
+
+/* Per-baseline coherency sum with shapes, gains, flags, residual and
+ * per-visibility chi^2 terms, honouring the source-batch extents
+ * (read the accumulator when nsrc.lower_extent > 0, write the partial sum
+ * and stop when nsrc.upper_extent < nsrc.global_size).
+ * Replaces rime_sum_coherencies (RimeSumCoherencies.py:405-429).
+ * gauss_shape / sersic_shape may be NULL when that type is empty;
+ * weight_vector may be NULL without MBX_APPLY_WEIGHTS. */
+int mbx_sum_coherencies_f32(const mbx_dims *d, const float *uvw,
+                            const float *gauss_shape, const float *sersic_shape,
+                            const float *frequency, const int32_t *antenna1,
+                            const int32_t *antenna2, const void *jones,
+                            const uint8_t *flag, const float *weight_vector,
+                            const void *observed_vis, const void *G_term,
+                            void *model_vis, float *chi_sqrd_result,
+                            const mbx_consts *k, int options, void *stream);
+int mbx_sum_coherencies_f64(const mbx_dims *d, const double *uvw,
+                            const double *gauss_shape, const double *sersic_shape,
+                            const double *frequency, const int32_t *antenna1,
+                            const int32_t *antenna2, const void *jones,
+                            const uint8_t *flag, const double *weight_vector,
+                            const void *observed_vis, const void *G_term,
+                            void *model_vis, double *chi_sqrd_result,
+                            const mbx_consts *k, int options, void *stream);
+
+/* X2 = sum of chi_sqrd_result over the active (time, baseline, channel)
+ * extents, divided by sigma_sqrd unless MBX_APPLY_WEIGHTS; accumulated in
+ * float64 and written to the device double *X2.
+ * Replaces pycuda gpuarray.sum + set_X2 (RimeSumCoherencies.py:531-536)
+ * and the v5 RimeReduction node (v5/gpu/RimeReduction.py:46-61).
+ * workspace: >= mbx_chi2_workspace_bytes() bytes of device memory. */
+size_t mbx_chi2_workspace_bytes(void);
+int mbx_chi2_reduce_f32(const mbx_dims *d, const float *chi_sqrd_result, double *X2,
+                        const mbx_consts *k, int options, void *workspace,
+                        size_t workspace_bytes, void *stream);
+int mbx_chi2_reduce_f64(const mbx_dims *d, const double *chi_sqrd_result, double *X2,
+                        const mbx_consts *k, int options, void *workspace,
+                        size_t workspace_bytes, void *stream);
+
+/* ---- Fused forward (no reference counterpart: replaces the whole v4
+ * pipeline RimeBSqrt -> RimeEBeam -> RimeEKBSqrt -> RimeSumCoherencies ->
+ * reduction, v4/RimeSolver.py:36-40, without the jones / B_sqrt scratch) */
+
+/* Inputs of the whole chain, all device pointers in v4 layouts. */
+typedef struct mbx_inputs {
+    const void *uvw, *lm, *stokes, *alpha, *frequency, *ref_frequency;
+    const void *parallactic_angles, *point_errors, *antenna_scaling, *E_beam;
+    const void *gauss_shape, *sersic_shape;     /* may be NULL if empty */
+    const int32_t *antenna1, *antenna2;
+    const uint8_t *flag;
+    const void *weight_vector;                  /* may be NULL w/o weights */
+    const void *observed_vis, *G_term;
+} mbx_inputs;
+
+/* Outputs. ekb_vis (optional, may be NULL) receives the source sum before
+ * gains, (ntime, nbl, nchan, 4) complex: the adjoint needs it. */
+typedef struct mbx_outputs {
+    void *model_vis;           /* (ntime, nbl, nchan, 4) complex          */
+    void *chi_sqrd_result;     /* (ntime, nbl, nchan) real                */
+    double *X2;                /* device scalar                           */
+    void *ekb_vis;             /* optional                                */
+} mbx_outputs;
+
+size_t mbx_forward_workspace_bytes(const mbx_dims *d, int is_double);
+int mbx_forward_f32(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                    const mbx_beam_extents *beam, const mbx_consts *k, int options,
+                    void *workspace, size_t workspace_bytes, void *stream);
+int mbx_forward_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                    const mbx_beam_extents *beam, const mbx_consts *k, int options,
+                    void *workspace, size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MONTBLANC_AMD_H */

@@ -1,0 +1,147 @@
+"""ctypes binding of the gfx950 RIME library (include/montblanc_amd.h).
+
+The shared library is built in-tree (montblanc_amd/libmontblanc_amd.so) by
+``montblanc_amd/csrc/Makefile``. There is no fallback: if the library is
+missing every entry point raises, so nothing silently runs on the CPU.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libmontblanc_amd.so')
+
+ABI_VERSION = 1
+APPLY_WEIGHTS = 1
+RESIDUAL_OUTPUT = 2
+
+ERROR_NAMES = {
+    -1: 'MBX_E_NULL', -2: 'MBX_E_DIMS', -3: 'MBX_E_EXTENT',
+    -4: 'MBX_E_WORKSPACE', -5: 'MBX_E_UNSUPPORTED', -6: 'MBX_E_NO_DEVICE',
+}
+
+
+class MontblancHipError(RuntimeError):
+    """A C-ABI entry point returned a non-zero status."""
+
+    def __init__(self, code, where, message):
+        self.code = code
+        name = ERROR_NAMES.get(code, 'hipError_t %d' % code if code > 0 else str(code))
+        super().__init__('%s failed (%s): %s' % (where, name, message))
+
+
+class mbx_dim(ctypes.Structure):
+    _fields_ = [('global_size', ctypes.c_uint32), ('local_size', ctypes.c_uint32),
+                ('lower_extent', ctypes.c_uint32), ('upper_extent', ctypes.c_uint32)]
+
+
+DIM_NAMES = ('ntime', 'na', 'nbl', 'nchan', 'npol', 'npolchan',
+             'nsrc', 'npsrc', 'ngsrc', 'nssrc', 'beam_lw', 'beam_mh', 'beam_nud')
+
+
+class mbx_dims(ctypes.Structure):
+    _fields_ = [(n, mbx_dim) for n in DIM_NAMES]
+
+
+class mbx_beam_extents(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ('ll', 'lm', 'lfreq', 'ul', 'um', 'ufreq')]
+
+
+class mbx_consts(ctypes.Structure):
+    _fields_ = [('gauss_scale', ctypes.c_double), ('two_pi_over_c', ctypes.c_double),
+                ('sigma_sqrd', ctypes.c_double)]
+
+
+INPUT_NAMES = ('uvw', 'lm', 'stokes', 'alpha', 'frequency', 'ref_frequency',
+               'parallactic_angles', 'point_errors', 'antenna_scaling', 'E_beam',
+               'gauss_shape', 'sersic_shape', 'antenna1', 'antenna2', 'flag',
+               'weight_vector', 'observed_vis', 'G_term')
+
+
+class mbx_inputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in INPUT_NAMES]
+
+
+OUTPUT_NAMES = ('model_vis', 'chi_sqrd_result', 'X2', 'ekb_vis')
+
+
+class mbx_outputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in OUTPUT_NAMES]
+
+
+_P = ctypes.c_void_p
+_PD = ctypes.POINTER(mbx_dims)
+_PB = ctypes.POINTER(mbx_beam_extents)
+_PK = ctypes.POINTER(mbx_consts)
+_PI = ctypes.POINTER(mbx_inputs)
+_PO = ctypes.POINTER(mbx_outputs)
+_I = ctypes.c_int
+_SZ = ctypes.c_size_t
+
+SIGNATURES = {
+    'mbx_abi_version': ([], _I),
+    'mbx_last_error': ([], ctypes.c_char_p),
+    'mbx_device_count': ([], _I),
+    'mbx_chi2_workspace_bytes': ([], _SZ),
+    'mbx_forward_workspace_bytes': ([_PD, _I], _SZ),
+}
+for _t in ('f32', 'f64'):
+    SIGNATURES.update({
+        'mbx_b_sqrt_' + _t: ([_PD, _P, _P, _P, _P, _P, _P], _I),
+        'mbx_e_beam_' + _t: ([_PD, _P, _P, _P, _P, _P, _P, _P, _PB, _P], _I),
+        'mbx_ekb_sqrt_' + _t: ([_PD, _P, _P, _P, _P, _P, _PK, _P], _I),
+        'mbx_sum_coherencies_' + _t: ([_PD] + [_P] * 13 + [_PK, _I, _P], _I),
+        'mbx_chi2_reduce_' + _t: ([_PD, _P, _P, _PK, _I, _P, _SZ, _P], _I),
+        'mbx_forward_' + _t: ([_PD, _PI, _PO, _PB, _PK, _I, _P, _SZ, _P], _I),
+    })
+
+_lib = None
+
+
+def load():
+    """Load the library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            'montblanc_amd: %s is missing; build it with '
+            '`make -C montblanc_amd/csrc` (hipcc, gfx950). There is no CPU '
+            'fallback.' % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    v = lib.mbx_abi_version()
+    if v != ABI_VERSION:
+        raise ImportError('montblanc_amd: ABI version %d, expected %d' % (v, ABI_VERSION))
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call entry point ``name``; raise MontblancHipError on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.mbx_last_error()
+        raise MontblancHipError(rc, name, msg.decode() if msg else '')
+    return rc
+
+
+def make_dims(sizes, extents=None, global_sizes=None):
+    """Build an mbx_dims from a dict of local sizes.
+
+    extents: optional dict name -> (lower, upper); default (0, local).
+    global_sizes: optional dict name -> global size; default local.
+    """
+    d = mbx_dims()
+    extents = extents or {}
+    global_sizes = global_sizes or {}
+    for n in DIM_NAMES:
+        local = int(sizes.get(n, 0))
+        lo, hi = extents.get(n, (0, local))
+        g = int(global_sizes.get(n, max(local, hi)))
+        setattr(d, n, mbx_dim(g, local, int(lo), int(hi)))
+    return d
