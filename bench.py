@@ -102,7 +102,8 @@ def cpu_baseline(w):
     first 16 channels, all sources and baselines (~10-30 s single core)."""
     import numpy as np
     from oracle import rime_np, synth
-    sample = dict(w, ntime=1, nchan=16)
+    sample = {k: v for k, v in w.items() if k != 'name'}
+    sample.update(ntime=1, nchan=16)
     a = synth.make_inputs(**sample, seed=1001)
     t0 = time.perf_counter()
     rime_np.rime(a, w['npsrc'], w['ngsrc'], w['nssrc'], src_chunk=64)
