@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 1
+#define MBX_ABI_VERSION 2
 
 enum {
     MBX_OK = 0,
@@ -175,12 +175,39 @@ typedef struct mbx_outputs {
     void *ekb_vis;             /* optional                                */
 } mbx_outputs;
 
+/* Pair-block plan: the baselines grouped into blocks of pb x qb antenna
+ * pairs (p0 + i, q0 + j) so that one thread reuses each per-antenna Jones
+ * pb (or qb) times. blocks is a DEVICE array of nblocks entries of
+ * (2 + pb*qb) int32: p0, q0, then the baseline index of slot i*qb + j or -1.
+ * Supported shapes: 1x1, 2x2 and 2x4. A NULL plan (or
+ * blocks == NULL) selects the generic kernel, which reads antenna1/2 per
+ * time step and so also handles layouts that change with time. */
+typedef struct mbx_pair_plan {
+    const int32_t *blocks;
+    int32_t nblocks, pb, qb, reserved;
+} mbx_pair_plan;
+
+/* Host-side plan builder (no device work). antenna1/antenna2 are HOST
+ * arrays (ntime x nbl int32). Writes at most `capacity` int32 into `out`
+ * and returns the number of blocks, or MBX_E_UNSUPPORTED when the layout
+ * differs between time steps (use the generic kernel), MBX_E_DIMS for an
+ * antenna index outside [0, na), MBX_E_WORKSPACE when capacity is short.
+ * mbx_pair_plan_capacity() is a sufficient capacity. No reference
+ * counterpart: the reference pairs antennas per baseline in each thread
+ * (RimeSumCoherencies.py:154-157). */
+int64_t mbx_pair_plan_capacity(int32_t nbl, int32_t pb, int32_t qb);
+int64_t mbx_build_pair_plan(const int32_t *antenna1, const int32_t *antenna2, int32_t ntime,
+                            int32_t nbl, int32_t na, int32_t pb, int32_t qb, int32_t *out,
+                            int64_t capacity);
+
 size_t mbx_forward_workspace_bytes(const mbx_dims *d, int is_double);
 int mbx_forward_f32(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
-                    const mbx_beam_extents *beam, const mbx_consts *k, int options,
+                    const mbx_beam_extents *beam, const mbx_consts *k,
+                    const mbx_pair_plan *plan, int options,
                     void *workspace, size_t workspace_bytes, void *stream);
 int mbx_forward_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
-                    const mbx_beam_extents *beam, const mbx_consts *k, int options,
+                    const mbx_beam_extents *beam, const mbx_consts *k,
+                    const mbx_pair_plan *plan, int options,
                     void *workspace, size_t workspace_bytes, void *stream);
 
 #ifdef __cplusplus

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libmontblanc_amd.so')
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 APPLY_WEIGHTS = 1
 RESIDUAL_OUTPUT = 2
 
@@ -69,6 +69,11 @@ class mbx_outputs(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in OUTPUT_NAMES]
 
 
+class mbx_pair_plan(ctypes.Structure):
+    _fields_ = [('blocks', ctypes.c_void_p), ('nblocks', ctypes.c_int32),
+                ('pb', ctypes.c_int32), ('qb', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _PD = ctypes.POINTER(mbx_dims)
 _PB = ctypes.POINTER(mbx_beam_extents)
@@ -77,6 +82,9 @@ _PI = ctypes.POINTER(mbx_inputs)
 _PO = ctypes.POINTER(mbx_outputs)
 _I = ctypes.c_int
 _SZ = ctypes.c_size_t
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_PP = ctypes.POINTER(mbx_pair_plan)
 
 SIGNATURES = {
     'mbx_abi_version': ([], _I),
@@ -84,6 +92,8 @@ SIGNATURES = {
     'mbx_device_count': ([], _I),
     'mbx_chi2_workspace_bytes': ([], _SZ),
     'mbx_forward_workspace_bytes': ([_PD, _I], _SZ),
+    'mbx_pair_plan_capacity': ([_I32, _I32, _I32], _I64),
+    'mbx_build_pair_plan': ([_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64], _I64),
 }
 for _t in ('f32', 'f64'):
     SIGNATURES.update({
@@ -92,7 +102,7 @@ for _t in ('f32', 'f64'):
         'mbx_ekb_sqrt_' + _t: ([_PD, _P, _P, _P, _P, _P, _PK, _P], _I),
         'mbx_sum_coherencies_' + _t: ([_PD] + [_P] * 13 + [_PK, _I, _P], _I),
         'mbx_chi2_reduce_' + _t: ([_PD, _P, _P, _PK, _I, _P, _SZ, _P], _I),
-        'mbx_forward_' + _t: ([_PD, _PI, _PO, _PB, _PK, _I, _P, _SZ, _P], _I),
+        'mbx_forward_' + _t: ([_PD, _PI, _PO, _PB, _PK, _PP, _I, _P, _SZ, _P], _I),
     })
 
 _lib = None
@@ -145,3 +155,22 @@ def make_dims(sizes, extents=None, global_sizes=None):
         g = int(global_sizes.get(n, max(local, hi)))
         setattr(d, n, mbx_dim(g, local, int(lo), int(hi)))
     return d
+
+
+def build_pair_plan(antenna1, antenna2, na, pb, qb):
+    """Pair-block plan (host int32 array) of a (ntime, nbl) antenna layout,
+    or None when the layout changes with time (generic kernel)."""
+    import numpy as np
+    a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
+    a2 = np.ascontiguousarray(antenna2, dtype=np.int32)
+    ntime, nbl = a1.shape
+    lib = load()
+    cap = lib.mbx_pair_plan_capacity(nbl, pb, qb)
+    out = np.empty(max(int(cap), 1), dtype=np.int32)
+    n = lib.mbx_build_pair_plan(a1.ctypes.data, a2.ctypes.data, ntime, nbl, na, pb, qb,
+                                out.ctypes.data, cap)
+    if n == -5:          # MBX_E_UNSUPPORTED: layout varies with time
+        return None
+    if n < 0:
+        raise MontblancHipError(int(n), 'mbx_build_pair_plan', lib.mbx_last_error().decode())
+    return out[: n * (2 + pb * qb)].reshape(n, 2 + pb * qb)

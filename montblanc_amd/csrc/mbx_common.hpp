@@ -134,8 +134,7 @@ __host__ __device__ __forceinline__ uint32_t dext(const mbx_dim &x) {
 // s = sqrt(det B), t = sqrt(trace B + 2 s); the matrix is left undivided
 // when t == 0 (then B == 0). CPUSolver.py:235-317, brightness.cuh:108-164.
 template <typename T>
-__device__ __forceinline__ J4<T> brightness_sqrt(T I, T Q, T U, T V, T alpha, T f, T rf) {
-    T p = pow(f / rf, alpha);
+__device__ __forceinline__ J4<T> brightness_sqrt_scaled(T I, T Q, T U, T V, T p) {
     I *= p; Q *= p; U *= p; V *= p;
     T s = sqrt(I * I - Q * Q - U * U - V * V);
     T t = sqrt(T(2) * I + T(2) * s);
@@ -149,6 +148,11 @@ __device__ __forceinline__ J4<T> brightness_sqrt(T I, T Q, T U, T V, T alpha, T 
         B = jscale(B, r);
     }
     return B;
+}
+
+template <typename T>
+__device__ __forceinline__ J4<T> brightness_sqrt(T I, T Q, T U, T V, T alpha, T f, T rf) {
+    return brightness_sqrt_scaled<T>(I, Q, U, V, pow(f / rf, alpha));
 }
 
 // Scalar phase term exp(i*phi), phi = -2 pi f / C (u l + v m + w (n - 1)),

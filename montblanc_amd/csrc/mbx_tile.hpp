@@ -1,0 +1,367 @@
+// Device machinery shared by the fused RIME kernels (forward, adjoint,
+// batched): argument block, LDS tile layout, the per-source and
+// per-antenna phases, shape factors and the visibility epilogue.
+//
+// A workgroup owns one time step t and a tile of `ct` channels. For each
+// tile of `st` sources it stages in LDS
+//   phase 0: source parameters (l, m, n-1, shape) and sqrt(B) per
+//            (source, channel)                      [CPUSolver.py:235-317]
+//   phase 1: per-antenna Jones J = E . (K sqrt(B)) per (source, antenna,
+//            channel)               [CPUSolver.py:162-233, 319-475, 543-560]
+// and the kernel-specific phase 2 consumes the tile.
+#pragma once
+
+#include "mbx_common.hpp"
+
+namespace mbx {
+
+// Prepared beam cube entry: value and amplitude of one (l, m, nu, corr).
+template <typename T> struct BeamTap { T re, im, amp, pad; };
+
+template <typename T> struct RimeArgs {
+    mbx_dims d;
+    const T *uvw, *lm, *stokes, *alpha, *freq, *ref_freq, *pa, *pe, *ascale, *gauss, *sersic, *wv;
+    const BeamTap<T> *beam;
+    const int32_t *a1, *a2;
+    const uint8_t *flag;
+    const cx<T> *obs, *G;
+    cx<T> *model;
+    T *chi2;
+    cx<T> *ekb;
+    double *partial;
+    double ll, lmin, lscale, mscale, lfreq, fscale;
+    double gauss_scale, two_pi_over_c;
+    int options;
+    int ct, st, nbchunk, bchunk, nctile;
+    const int32_t *blocks;    // pair-block plan (blocked kernels)
+    int nblocks, pb, qb;
+};
+
+// LDS carve-up of one tile.
+constexpr int CH = 5;   // doubles per channel in the tile
+
+template <typename T> struct Tile {
+    double *uvw;     // [NA][3]
+    double *pa;      // [NA][2]  sin, cos of the parallactic angle
+    double *ch;      // [ct][CH] gc0, gc1, chd, frequency, log(frequency / ref)
+    double *src;     // [st][3]  l, m, n - 1
+    T *shape;        // [st][4]  p0, p1, p2, type (0 point, 1 gaussian, 2 sersic)
+    J4<T> *sqB;      // [st][ct]
+    J4<T> *J;        // [st][NA][ct]
+    double *part;    // [nwaves]
+
+    __device__ __forceinline__ Tile(unsigned char *base, int NA, int ct, int st) {
+        uvw = reinterpret_cast<double *>(base);
+        pa = uvw + 3 * NA;
+        ch = pa + 2 * NA;
+        src = ch + CH * ct;
+        shape = reinterpret_cast<T *>(src + 3 * st);
+        uintptr_t off = reinterpret_cast<uintptr_t>(shape + 4 * st);
+        off = (off + 15) & ~uintptr_t(15);
+        sqB = reinterpret_cast<J4<T> *>(off);
+        J = sqB + st * ct;
+        part = reinterpret_cast<double *>(J + st * NA * ct);
+    }
+};
+
+template <typename T>
+static inline size_t tile_lds_bytes(int na, int ct, int st, int nwaves) {
+    return sizeof(double) * (5 * size_t(na) + CH * ct + 3 * st) + sizeof(T) * 4 * st + 16 +
+           sizeof(J4<T>) * size_t(st) * ct * (1 + size_t(na)) + sizeof(double) * nwaves;
+}
+
+// exp(i phi) with phi evaluated in float64.
+//
+// Method (textbook, not a library's): Cody-Waite reduction of phi by pi/2
+// in float64 with a two-term pi/2 (W. J. Cody, W. Waite, "Software Manual
+// for the Elementary Functions", 1980) to y in [-pi/4, pi/4] and quadrant
+// q, then the Taylor series of sin and cos on y (coefficients 1/n!), and a
+// quadrant rotation. Degree 17/18 in float64 (truncation < 1e-19 relative),
+// degree 9/10 in float (truncation < 2e-9). Keeps the kernels free of the
+// general-argument paths of a library sincos. Accuracy is tested through
+// the phase term against numpy (tests/test_gpu_*).
+__device__ __forceinline__ double reduce_half_pi(double phi, int &quadrant) {
+    const double two_over_pi = 0.6366197723675814;
+    const double pio2_hi = 1.5707963267948966;       // pi/2 rounded to double
+    const double pio2_lo = 6.123233995736766e-17;    // pi/2 - pio2_hi
+    double q = rint(phi * two_over_pi);
+    quadrant = int(int64_t(q) & 3);
+    return fma(-q, pio2_lo, fma(-q, pio2_hi, phi));
+}
+
+__device__ __forceinline__ void taylor_sincos(double y, double *s, double *c) {
+    const double y2 = y * y;
+    // sin y = y (1 - y^2/3! + y^4/5! - ... - y^16/17!)
+    double ps = -1.0 / 355687428096000.0;            // -1/17!
+    ps = fma(ps, y2, 1.0 / 1307674368000.0);         //  1/15!
+    ps = fma(ps, y2, -1.0 / 6227020800.0);           // -1/13!
+    ps = fma(ps, y2, 1.0 / 39916800.0);              //  1/11!
+    ps = fma(ps, y2, -1.0 / 362880.0);               // -1/9!
+    ps = fma(ps, y2, 1.0 / 5040.0);                  //  1/7!
+    ps = fma(ps, y2, -1.0 / 120.0);                  // -1/5!
+    ps = fma(ps, y2, 1.0 / 6.0);                     //  1/3!
+    *s = fma(-y * y2, ps, y);
+    // cos y = 1 - y^2/2! + y^4/4! - ... + y^18/18!
+    double pc = 1.0 / 6402373705728000.0;            //  1/18!
+    pc = fma(pc, y2, -1.0 / 20922789888000.0);       // -1/16!
+    pc = fma(pc, y2, 1.0 / 87178291200.0);           //  1/14!
+    pc = fma(pc, y2, -1.0 / 479001600.0);            // -1/12!
+    pc = fma(pc, y2, 1.0 / 3628800.0);               //  1/10!
+    pc = fma(pc, y2, -1.0 / 40320.0);                // -1/8!
+    pc = fma(pc, y2, 1.0 / 720.0);                   //  1/6!
+    pc = fma(pc, y2, -1.0 / 24.0);                   // -1/4!
+    pc = fma(pc, y2, 0.5);                           //  1/2!
+    *c = fma(-y2, pc, 1.0);
+}
+
+__device__ __forceinline__ void taylor_sincos(float y, float *s, float *c) {
+    const float y2 = y * y;
+    float ps = 1.0f / 362880.0f;                     //  1/9!
+    ps = fmaf(ps, y2, -1.0f / 5040.0f);
+    ps = fmaf(ps, y2, 1.0f / 120.0f);
+    ps = fmaf(ps, y2, -1.0f / 6.0f);
+    *s = fmaf(y * y2, ps, y);
+    float pc = -1.0f / 3628800.0f;                   // -1/10!
+    pc = fmaf(pc, y2, 1.0f / 40320.0f);
+    pc = fmaf(pc, y2, -1.0f / 720.0f);
+    pc = fmaf(pc, y2, 1.0f / 24.0f);
+    pc = fmaf(pc, y2, -0.5f);
+    *c = fmaf(y2, pc, 1.0f);
+}
+
+// (sin, cos) of phi given the quadrant rotation of the reduced argument.
+template <typename T>
+__device__ __forceinline__ void sincos_phase(double phi, T *s, T *c) {
+    int q;
+    const double y = reduce_half_pi(phi, q);
+    T sy, cy;
+    taylor_sincos(T(y), &sy, &cy);
+    const T ss = (q & 1) ? cy : sy;
+    const T cc = (q & 1) ? sy : cy;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+template <typename T> struct PhaseCS {
+    __device__ __forceinline__ static void eval(double phi, T *s, T *c) { sincos_phase<T>(phi, s, c); }
+};
+
+template <typename T>
+__device__ __forceinline__ void tile_setup(const RimeArgs<T> &A, Tile<T> &L, int t, int c0,
+                                           int tid, int nth) {
+    const mbx_dims &d = A.d;
+    const int NA = d.na.local_size;
+    const int nc_ext = dext(d.nchan);
+    for (int a = tid; a < NA; a += nth) {
+        const T *x = A.uvw + (uint64_t(t) * NA + a) * 3;
+        L.uvw[3 * a] = x[0]; L.uvw[3 * a + 1] = x[1]; L.uvw[3 * a + 2] = x[2];
+        double sn, cs;
+        sincos_phase<double>(double(A.pa[uint64_t(t) * NA + a]), &sn, &cs);
+        L.pa[2 * a] = sn; L.pa[2 * a + 1] = cs;
+    }
+    const double NUD1 = double(d.beam_nud.local_size - 1);
+    for (int k = tid; k < A.ct; k += nth) {
+        int c = min(c0 + k, nc_ext - 1);
+        double f = A.freq[c];
+        double vc = fmin(NUD1, fmax(0.0, A.fscale * (f - A.lfreq)));
+        double g0 = floor(vc);
+        L.ch[CH * k] = g0;
+        L.ch[CH * k + 1] = fmin(g0 + 1.0, NUD1);
+        L.ch[CH * k + 2] = vc - g0;
+        L.ch[CH * k + 3] = f;
+        L.ch[CH * k + 4] = log(f / double(A.ref_freq[c]));
+    }
+}
+
+// Phase 0: parameters of sources [s0, s0 + nst) and sqrt(B) per channel.
+template <typename T>
+__device__ __forceinline__ void tile_phase0(const RimeArgs<T> &A, Tile<T> &L, int t, int c0,
+                                            int s0, int nst, int tid, int nth) {
+    const mbx_dims &d = A.d;
+    const uint64_t NT = d.ntime.local_size;
+    const int np = dext(d.npsrc), ng = dext(d.ngsrc);
+    const uint64_t NG = d.ngsrc.local_size, NS = d.nssrc.local_size;
+    const int nc_ext = dext(d.nchan);
+    const int ct = A.ct;
+    for (int i = tid; i < nst; i += nth) {
+        int s = s0 + i;
+        double l = A.lm[2 * uint64_t(s)], m = A.lm[2 * uint64_t(s) + 1];
+        L.src[3 * i] = l;
+        L.src[3 * i + 1] = m;
+        L.src[3 * i + 2] = sqrt(1.0 - l * l - m * m) - 1.0;
+        T p0 = T(0), p1 = T(0), p2 = T(0), ty = T(0);
+        if (s >= np && s < np + ng) {
+            int g = s - np;
+            p0 = A.gauss[g]; p1 = A.gauss[g + NG]; p2 = A.gauss[g + 2 * NG]; ty = T(1);
+        } else if (s >= np + ng) {
+            int g = s - np - ng;
+            p0 = A.sersic[g]; p1 = A.sersic[g + NS];
+            p2 = A.sersic[g + 2 * NS] / (T(1) - p0 * p0 - p1 * p1); ty = T(2);
+        }
+        L.shape[4 * i] = p0; L.shape[4 * i + 1] = p1; L.shape[4 * i + 2] = p2;
+        L.shape[4 * i + 3] = ty;
+    }
+    for (int i = tid; i < nst * ct; i += nth) {
+        int si = i / ct, k = i % ct;
+        int s = s0 + si;
+        int c = min(c0 + k, nc_ext - 1);
+        const T *S = A.stokes + (uint64_t(s) * NT + t) * 4;
+        (void)c;
+        const T power = T(exp(double(A.alpha[uint64_t(s) * NT + t]) * L.ch[CH * k + 4]));
+        L.sqB[i] = brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
+    }
+}
+
+// Beam Jones of one (source, antenna, channel) from the prepared cube
+// (RimeEBeam.py:233-343 / CPUSolver.py:348-475 semantics); grid
+// coordinates in float64.
+template <typename T>
+__device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const Tile<T> &L, int t, int a,
+                                            int c, int k, double l0, double m0) {
+    const mbx_dims &d = A.d;
+    const uint64_t NA = d.na.local_size, NC = d.nchan.local_size;
+    const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
+    const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
+    const double ps = L.pa[2 * a], pc = L.pa[2 * a + 1];
+    double l = (l0 * pc - m0 * ps + double(pet[0])) * double(asc[0]);
+    double m = (l0 * ps + m0 * pc + double(pet[1])) * double(asc[1]);
+    const double LW1 = double(d.beam_lw.local_size - 1), MH1 = double(d.beam_mh.local_size - 1);
+    double vl = fmin(LW1, fmax(0.0, A.lscale * (l - A.ll)));
+    double gl0 = floor(vl);
+    double vm = fmin(MH1, fmax(0.0, A.mscale * (m - A.lmin)));
+    double gm0 = floor(vm);
+    const T ld = T(vl - gl0), md = T(vm - gm0), chd = T(L.ch[CH * k + 2]);
+    const uint32_t MH = d.beam_mh.local_size, NUD = d.beam_nud.local_size;
+    const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
+    const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
+    const uint32_t ic0 = uint32_t(L.ch[CH * k]), ic1 = uint32_t(L.ch[CH * k + 1]);
+    const T wc0 = T(1) - chd, wc1 = chd;
+    cx<T> acc[4];
+    T amp[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) { acc[p] = {T(0), T(0)}; amp[p] = T(0); }
+    // Four (l, m) corners, one at a time (bounds the loads in flight);
+    // both channel taps of a corner are adjacent in the cube.
+#pragma unroll 1
+    for (int corner = 0; corner < 4; ++corner) {
+        const bool hl = corner & 1, hm = corner >> 1;
+        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0;
+        const T wlm = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md);
+        const BeamTap<T> *row = A.beam + (uint64_t(il * MH + im) * NUD) * 4;
+        const BeamTap<T> *t0 = row + ic0 * 4, *t1 = row + ic1 * 4;
+        const T w0 = wlm * wc0, w1 = wlm * wc1;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const BeamTap<T> e0 = t0[p], e1 = t1[p];
+            acc[p].re = fma(w1, e1.re, fma(w0, e0.re, acc[p].re));
+            acc[p].im = fma(w1, e1.im, fma(w0, e0.im, acc[p].im));
+            amp[p] = fma(w1, e1.amp, fma(w0, e0.amp, amp[p]));
+        }
+    }
+    J4<T> E;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        T n2 = acc[p].re * acc[p].re + acc[p].im * acc[p].im;
+        T g = (n2 > T(0) && isfinite(n2)) ? amp[p] / sqrt(n2) : amp[p];
+        E.v[p] = {acc[p].re * g, acc[p].im * g};
+    }
+    return E;
+}
+
+// Phase 1: J = E . (K sqrt(B)) for every (source, antenna, channel) of
+// the tile.
+template <typename T>
+__device__ __forceinline__ void tile_phase1(const RimeArgs<T> &A, Tile<T> &L, int t, int c0,
+                                            int nst, int tid, int nth) {
+    const int NA = A.d.na.local_size;
+    const int nc_ext = dext(A.d.nchan);
+    const int ct = A.ct;
+    const int n1 = nst * NA * ct;
+    for (int i = tid; i < n1; i += nth) {
+        const int k = i % ct;
+        const int a = (i / ct) % NA;
+        const int si = i / (ct * NA);
+        const int c = min(c0 + k, nc_ext - 1);
+        const double l0 = L.src[3 * si], m0 = L.src[3 * si + 1], nm1 = L.src[3 * si + 2];
+        J4<T> E = beam_jones<T>(A, L, t, a, c, k, l0, m0);
+        const double *x = L.uvw + 3 * a;
+        double p = x[2] * nm1 + x[1] * m0 + x[0] * l0;
+        double phi = -A.two_pi_over_c * L.ch[CH * k + 3] * p;
+        T sn, cs;
+        PhaseCS<T>::eval(phi, &sn, &cs);
+        const cx<T> K = {cs, sn};
+        const J4<T> &B = L.sqB[si * ct + k];
+        J4<T> KB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
+        L.J[i] = jmul(E, KB);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T gauss_f(T u, T v, T el, T em, T R, T k) {
+    T x = (u * em - v * el) * k * R;
+    T y = (u * el + v * em) * k;
+    return exp(-(x * x + y * y));
+}
+
+template <typename T>
+__device__ __forceinline__ T sersic_f(T u, T v, T e1, T e2, T Rp, T k) {
+    // Rp = R / (1 - e1^2 - e2^2), precomputed in phase 0
+    T x = (u * (T(1) + e1) + v * e2) * k * Rp;
+    T y = (u * e2 + v * (T(1) - e1)) * k * Rp;
+    T den = T(1) + x * x + y * y;
+    return T(1) / (den * sqrt(den));
+}
+
+// Epilogue of one visibility (t, b, c): M = G_p V G_q^H, flags,
+// model/residual output and the chi^2 term (RimeSumCoherencies.py:318-392).
+// Also stores V to ekb when requested. Returns the chi^2 term.
+template <typename T>
+__device__ __forceinline__ double vis_out(const RimeArgs<T> &A, int t, int b, int c, int p, int q,
+                                          const J4<T> &V, bool final_batch) {
+    const mbx_dims &d = A.d;
+    const uint64_t NA = d.na.local_size, NC = d.nchan.local_size, NBL = d.nbl.local_size;
+    const uint64_t vi = (uint64_t(t) * NBL + b) * NC + c;
+    if (A.ekb) jstore(A.ekb + vi * 4, V);
+    if (!final_batch) return 0.0;
+    const uint64_t gp = ((uint64_t(t) * NA + p) * NC + c) * 4;
+    const uint64_t gq = ((uint64_t(t) * NA + q) * NC + c) * 4;
+    J4<T> M = jmul_h(jmul(jload(A.G + gp), V), jload(A.G + gq));
+    J4<T> O = jload(A.obs + vi * 4);
+    uchar4 f4 = *reinterpret_cast<const uchar4 *>(A.flag + vi * 4);
+    const uint8_t fa[4] = {f4.x, f4.y, f4.z, f4.w};
+    J4<T> R;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (fa[k] > 0) { M.v[k] = {T(0), T(0)}; O.v[k] = {T(0), T(0)}; }
+        R.v[k] = {O.v[k].re - M.v[k].re, O.v[k].im - M.v[k].im};
+    }
+    jstore(A.model + vi * 4, (A.options & MBX_RESIDUAL_OUTPUT) ? R : M);
+    T term = T(0);
+    if (A.options & MBX_APPLY_WEIGHTS) {
+        const T *w = A.wv + vi * 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) term += (R.v[k].re * R.v[k].re + R.v[k].im * R.v[k].im) * w[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) term += R.v[k].re * R.v[k].re + R.v[k].im * R.v[k].im;
+    }
+    A.chi2[vi] = term;
+    return double(term);
+}
+
+// Workgroup sum of a float64 value (wave64 shuffles, then LDS); thread 0
+// writes it to partial[slot].
+__device__ __forceinline__ void wg_partial(double v, double *part, double *out, int tid, int nth) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((tid & 63) == 0) part[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (nth + 63) / 64; ++w) s += part[w];
+        *out = s;
+    }
+}
+
+}  // namespace mbx

@@ -170,8 +170,10 @@ class RimeFusedForward(_LibNode):
         ws = s.workspace('forward', nb)
         b = beam_extents(s)
         k = consts(s)
+        plan = s.pair_plan()
         _lib.call('mbx_forward_' + suffix(s), ctypes.byref(d), ctypes.byref(inputs),
-                  ctypes.byref(outs), ctypes.byref(b), ctypes.byref(k), options(s), ptr(ws), nb,
+                  ctypes.byref(outs), ctypes.byref(b), ctypes.byref(k),
+                  ctypes.byref(plan[0]) if plan is not None else None, options(s), ptr(ws), nb,
                   ctypes.c_void_p(stream_handle(s, stream)))
 
 

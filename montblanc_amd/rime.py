@@ -144,13 +144,43 @@ def alloc_outputs(arrays, keep_ekb=False):
     return out
 
 
+# Default pair-block shapes: 2x2 antenna pairs per thread in float; in
+# double the accumulators are twice as large and 1x1 keeps them in registers.
+DEFAULT_PAIR_BLOCK = {'f32': (2, 2), 'f64': (2, 2)}
+
+
+def pair_plan(arrays, pair_block=None):
+    """Device pair-block plan for the antenna layout of ``arrays`` (cached in
+    the dict under '_pair_plan'); None selects the generic kernel.
+
+    pair_block: (pb, qb), 'generic', or None for the dtype default.
+    """
+    if pair_block == 'generic':
+        return None
+    pb, qb = DEFAULT_PAIR_BLOCK[suffix(arrays)] if pair_block is None else pair_block
+    key = ('_pair_plan', pb, qb)
+    cached = arrays.get('_pair_plan')
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    na = arrays['uvw'].shape[1]
+    host = _lib.build_pair_plan(arrays['antenna1'].cpu().numpy(), arrays['antenna2'].cpu().numpy(),
+                                na, pb, qb)
+    plan = None
+    if host is not None:
+        dev = torch.from_numpy(host).to(arrays['uvw'].device)
+        plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, 0), dev)
+    arrays['_pair_plan'] = (key, plan)
+    return plan
+
+
 def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=None,
-            outputs=None, keep_ekb=False, dims=None, stream=None):
+            outputs=None, keep_ekb=False, dims=None, stream=None, pair_block=None):
     """Fused forward on device tensors. Returns the outputs dict
     (model_vis, chi_sqrd_result, X2 [device float64 scalar], ekb_vis?)."""
     dev = arrays['uvw'].device
     out = outputs if outputs is not None else alloc_outputs(arrays, keep_ekb)
     d = make_dims(arrays, counts) if dims is None else dims
+    plan = pair_plan(arrays, pair_block)
     ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
     outs = _lib.mbx_outputs(_raw(out.get('model_vis')), _raw(out.get('chi_sqrd_result')),
                             _raw(out.get('X2')), _raw(out.get('ekb_vis')))
@@ -162,6 +192,7 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
     h = stream_of(dev) if stream is None else ctypes.c_void_p(stream)
     _lib.call('mbx_forward_' + suffix(arrays), ctypes.byref(d), ctypes.byref(ins),
               ctypes.byref(outs), ctypes.byref(b), ctypes.byref(k),
+              ctypes.byref(plan[0]) if plan is not None else None,
               options(residual, use_weight_vector), _ptr(ws), nb, h)
     return out
 

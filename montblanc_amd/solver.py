@@ -102,6 +102,8 @@ class RimeSolver(object):
         self._workspaces = {}
         self._x2_dev = None
         self._x2_host = None
+        self._pair_plan = None
+        self._host_antenna = [None, None]
 
         self.register_default_dimensions()
         self.register_dimension('beam_lw', slvr_cfg[Options.E_BEAM_WIDTH], 'E cube l width')
@@ -346,6 +348,24 @@ class RimeSolver(object):
             setattr(self, 'transfer_' + name, (lambda ary, _n=name: self.transfer(_n, ary)))
             setattr(self, 'retrieve_' + name, (lambda _n=name: self.retrieve(_n)))
 
+    def pair_plan(self):
+        """Device pair-block plan of the current antenna layout (rebuilt after
+        antenna1/antenna2 transfers); None when the layout varies with time."""
+        if self._pair_plan is None:
+            from .rime import DEFAULT_PAIR_BLOCK
+            pb, qb = DEFAULT_PAIR_BLOCK['f32' if self.is_float() else 'f64']
+            for i, name in enumerate(('antenna1', 'antenna2')):
+                if self._host_antenna[i] is None:
+                    self._host_antenna[i] = self.retrieve(name)
+            host = _lib.build_pair_plan(self._host_antenna[0], self._host_antenna[1],
+                                        self.dim_local_size('na'), pb, qb)
+            plan = None
+            if host is not None:
+                dev = torch.from_numpy(host).to(self.device)
+                plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, 0), dev)
+            self._pair_plan = plan if plan is not None else False
+        return self._pair_plan or None
+
     def transfer(self, name, ary):
         """Host ndarray -> device array; shape and dtype must match (H2D set)."""
         a = self._arrays[name]
@@ -358,6 +378,10 @@ class RimeSolver(object):
         t = self.tensor(name)
         if t.numel():
             t.copy_(torch.from_numpy(np.ascontiguousarray(ary)))
+        if name in ('antenna1', 'antenna2'):
+            i = 0 if name == 'antenna1' else 1
+            self._host_antenna[i] = np.array(ary, dtype=np.int32, copy=True)
+            self._pair_plan = None
 
     def retrieve(self, name):
         """Device array -> host ndarray (D2H get)."""

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     from montblanc_amd import _lib
     lib = _lib.load()
     names = declared_symbols()
-    assert len(names) >= 17
+    assert len(names) >= 19
     for n in sorted(names):
         assert hasattr(lib, n), n
     assert set(_lib.SIGNATURES) == names
@@ -89,7 +89,7 @@ def test_validation_errors_without_launching():
     outs = _lib.mbx_outputs()
     b = _lib.mbx_beam_extents(-1, -1, 1, 1, 1, 2)
     rc = lib.mbx_forward_f64(ctypes.byref(dims_for()), ctypes.byref(ins), ctypes.byref(outs),
-                             ctypes.byref(b), ctypes.byref(k), 0, fake, 1 << 30, None)
+                             ctypes.byref(b), ctypes.byref(k), None, 0, fake, 1 << 30, None)
     assert rc == -1
 
 
@@ -194,3 +194,42 @@ def test_pipeline_swallows_errors_and_keeps_going_on_shutdown():
     with pytest.raises(NotImplementedError):
         Node().execute(None)
     assert str(Pipeline([Ok(), Ok()])) == 'Ok Ok'
+
+
+# ---- pair-block planner (host code of the library) ------------------------
+
+@pytest.mark.parametrize('na,pb,qb,auto', [(64, 2, 2, False), (27, 2, 4, False), (7, 1, 1, False),
+                                           (9, 2, 2, True), (5, 2, 4, True)])
+def test_pair_plan_covers_every_baseline_once(na, pb, qb, auto):
+    from montblanc_amd import _lib
+    from oracle import rime_np
+    a1, a2 = rime_np.default_ant_pairs(na, 3, auto_correlations=auto)
+    plan = _lib.build_pair_plan(a1, a2, na, pb, qb)
+    nbl = a1.shape[1]
+    seen = np.zeros(nbl, int)
+    for row in plan:
+        p0, q0 = row[0], row[1]
+        assert p0 % pb == 0 and q0 % qb == 0
+        for s, b in enumerate(row[2:]):
+            if b < 0:
+                continue
+            i, j = divmod(s, qb)
+            assert (a1[0, b], a2[0, b]) == (p0 + i, q0 + j)
+            seen[b] += 1
+    assert np.all(seen == 1)
+    # blocks are ordered by (p0, q0): neighbouring threads share antennas
+    keys = [tuple(r[:2]) for r in plan]
+    assert keys == sorted(keys)
+
+
+def test_pair_plan_duplicates_time_variation_and_bad_indices():
+    from montblanc_amd import _lib
+    a1 = np.array([[0, 0, 1, 0]], np.int32)
+    a2 = np.array([[1, 2, 2, 1]], np.int32)        # baseline (0, 1) twice
+    plan = _lib.build_pair_plan(a1, a2, 3, 2, 2)
+    got = sorted(int(b) for b in plan[:, 2:].ravel() if b >= 0)
+    assert got == [0, 1, 2, 3]
+    varying = np.array([[0, 1], [1, 0]], np.int32)
+    assert _lib.build_pair_plan(varying, varying + 1, 3, 2, 2) is None
+    with pytest.raises(_lib.MontblancHipError):
+        _lib.build_pair_plan(a1, a2 + 5, 3, 2, 2)
