@@ -184,7 +184,12 @@ typedef struct mbx_outputs {
  * time step and so also handles layouts that change with time. */
 typedef struct mbx_pair_plan {
     const int32_t *blocks;
-    int32_t nblocks, pb, qb, reserved;
+    int32_t nblocks, pb, qb;
+    /* Kernel form: >0 = double-buffered kernel with that many producer
+     * waves building the next source tile while the consumer waves
+     * accumulate; 0 = the same with an automatic producer count;
+     * <0 = single-role kernel (every wave does both phases). */
+    int32_t producer_waves;
 } mbx_pair_plan;
 
 /* Host-side plan builder (no device work). antenna1/antenna2 are HOST

@@ -71,7 +71,8 @@ class mbx_outputs(ctypes.Structure):
 
 class mbx_pair_plan(ctypes.Structure):
     _fields_ = [('blocks', ctypes.c_void_p), ('nblocks', ctypes.c_int32),
-                ('pb', ctypes.c_int32), ('qb', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+                ('pb', ctypes.c_int32), ('qb', ctypes.c_int32),
+                ('producer_waves', ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

@@ -262,11 +262,156 @@ k_forward_blk(RimeArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------
+// Double-buffered pair-block kernel. Waves [0, cth/64) are consumers (one
+// pair block and channel each, accumulators in registers); the remaining
+// waves are producers that build the Jones tile of the next source tile
+// in the other LDS buffer while the consumers work on the current one. One
+// barrier per source tile; neither role holds the other's registers.
+
+template <typename T> constexpr int pc_max_threads() { return sizeof(T) == 4 ? 1024 : 768; }
+
+template <typename T, int PB, int QB>
+__device__ __forceinline__ void pc_consume(const RimeArgs<T> &A, const PcTile<T> &L, int buf,
+                                           int nst, int k, const int *pi, const int *qj,
+                                           const T *up, const T *vp, const T *uq, const T *vq,
+                                           J4<T> (&acc)[PB][QB]) {
+    const int NA = A.d.na.local_size;
+    const int ct = A.ct;
+    const J4<T> *Jb = L.J + buf * L.jstride + k;
+    const T *shape = L.shape + buf * 4 * A.st;
+    const T kf_g = T(A.gauss_scale) * T(L.ch[CH * k + 3]);
+    const T kf_s = T(A.two_pi_over_c) * T(L.ch[CH * k + 3]);
+    for (int si = 0; si < nst; ++si) {
+        const J4<T> *Js = Jb + si * NA * ct;
+        const T ty = shape[4 * si + 3];
+        J4<T> Jp[PB];
+#pragma unroll
+        for (int i = 0; i < PB; ++i) Jp[i] = Js[pi[i] * ct];
+        if (ty == T(0)) {
+#pragma unroll
+            for (int j = 0; j < QB; ++j) {
+                const J4<T> Jq = Js[qj[j] * ct];
+#pragma unroll
+                for (int i = 0; i < PB; ++i) jacc_h(acc[i][j], Jp[i], Jq);
+            }
+        } else {
+            const T s0p = shape[4 * si], s1p = shape[4 * si + 1], s2p = shape[4 * si + 2];
+            T f[PB][QB];
+            if (ty == T(1)) {
+#pragma unroll
+                for (int i = 0; i < PB; ++i)
+#pragma unroll
+                    for (int j = 0; j < QB; ++j)
+                        f[i][j] = gauss_f(uq[j] - up[i], vq[j] - vp[i], s0p, s1p, s2p, kf_g);
+            } else {
+#pragma unroll
+                for (int i = 0; i < PB; ++i)
+#pragma unroll
+                    for (int j = 0; j < QB; ++j)
+                        f[i][j] = sersic_f(uq[j] - up[i], vq[j] - vp[i], s0p, s1p, s2p, kf_s);
+            }
+#pragma unroll
+            for (int j = 0; j < QB; ++j) {
+                const J4<T> Jq = Js[qj[j] * ct];
+#pragma unroll
+                for (int i = 0; i < PB; ++i) jacc_h(acc[i][j], jscale(Jp[i], f[i][j]), Jq);
+            }
+        }
+    }
+}
+
+template <typename T, int PB, int QB>
+__global__ void __launch_bounds__(pc_max_threads<T>())
+k_forward_pc(RimeArgs<T> A, int cth) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const mbx_dims &d = A.d;
+    const int NA = d.na.local_size;
+    const int ct = A.ct, st = A.st;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    int bc, ctile, t;
+    wg_coords(A, bc, ctile, t);
+    PcTile<T> L(smem_raw, NA, ct, st);
+    const bool active = t < int(dext(d.ntime));
+    const int c0 = ctile * ct;
+    if (active) tile_setup<T>(A, L.uvw, L.pa, L.ch, t, c0, tid, nth);
+    __syncthreads();
+    const int ns_all = dext(d.nsrc);
+    const int ntiles = (ns_all + st - 1) / st;
+    // The role is a per-wave (scalar) decision, so each wave runs exactly
+    // one of the two loops below; both execute 1 + ntiles barriers.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double term_sum = 0.0;
+    if (wave < (cth >> 6)) {
+        const uint64_t NC = d.nchan.local_size, NBL = d.nbl.local_size;
+        const int nc_ext = dext(d.nchan);
+        const int blk_lo = bc * A.bchunk;
+        const int nbk = max(0, min(A.bchunk, A.nblocks - blk_lo));
+        const bool mine = active && tid < nbk * ct;
+        const int blk = blk_lo + (mine ? tid / ct : 0);
+        const int k = mine ? tid % ct : 0;
+        const bool chan_ok = c0 + k < nc_ext;
+        constexpr int W = 2 + PB * QB;
+        const int32_t *plan = A.blocks + uint64_t(min(blk, A.nblocks - 1)) * W;
+        int pi[PB], qj[QB];
+        const int p0 = plan[0], q0 = plan[1];
+#pragma unroll
+        for (int i = 0; i < PB; ++i) pi[i] = min(max(p0 + i, 0), NA - 1);
+#pragma unroll
+        for (int j = 0; j < QB; ++j) qj[j] = min(max(q0 + j, 0), NA - 1);
+        J4<T> acc[PB][QB];
+#pragma unroll
+        for (int i = 0; i < PB; ++i)
+#pragma unroll
+            for (int j = 0; j < QB; ++j) {
+                acc[i][j] = jzero<T>();
+                const int b = mine ? plan[2 + i * QB + j] : -1;
+                if (b >= 0 && chan_ok && d.nsrc.lower_extent > 0)
+                    acc[i][j] = jload(A.ekb + ((uint64_t(t) * NBL + b) * NC + c0 + k) * 4);
+            }
+        T up[PB], vp[PB], uq[QB], vq[QB];
+#pragma unroll
+        for (int i = 0; i < PB; ++i) { up[i] = T(L.uvw[3 * pi[i]]); vp[i] = T(L.uvw[3 * pi[i] + 1]); }
+#pragma unroll
+        for (int j = 0; j < QB; ++j) { uq[j] = T(L.uvw[3 * qj[j]]); vq[j] = T(L.uvw[3 * qj[j] + 1]); }
+        __syncthreads();                       // tile 0 produced
+        for (int it = 0; it < ntiles; ++it) {
+            const int nst = min(st, ns_all - it * st);
+            if (mine) pc_consume<T, PB, QB>(A, L, it & 1, nst, k, pi, qj, up, vp, uq, vq, acc);
+            __syncthreads();
+        }
+        const bool final_batch = d.nsrc.upper_extent >= d.nsrc.global_size;
+        if (mine && chan_ok) {
+#pragma unroll
+            for (int i = 0; i < PB; ++i)
+#pragma unroll
+                for (int j = 0; j < QB; ++j) {
+                    const int b = plan[2 + i * QB + j];
+                    if (b >= 0)
+                        term_sum += vis_out(A, t, b, c0 + k, pi[i], qj[j], acc[i][j], final_batch);
+                    // one visibility at a time: keeps the epilogues' registers apart
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+        }
+    } else {
+        const int ptid = tid - cth, pth = nth - cth;
+        if (active && ntiles > 0) pc_produce<T>(A, L, 0, t, c0, 0, min(st, ns_all), ptid, pth);
+        __syncthreads();
+        for (int it = 0; it < ntiles; ++it) {
+            const int s1 = (it + 1) * st;
+            if (active && it + 1 < ntiles)
+                pc_produce<T>(A, L, (it + 1) & 1, t, c0, s1, min(st, ns_all - s1), ptid, pth);
+            __syncthreads();
+        }
+    }
+    wg_partial(term_sum, L.part, A.partial + blockIdx.x, tid, nth);
+}
+
+// ---------------------------------------------------------------------
 // Launch planning.
 
 struct FwdPlan {
-    bool blocked;
-    int ct, st, ni, nbchunk, bchunk, nctile, threads;
+    bool blocked, pc;
+    int ct, st, ni, nbchunk, bchunk, nctile, threads, cth;
     uint64_t nwg;
     size_t lds_bytes;
 };
@@ -285,6 +430,35 @@ static FwdPlan plan_forward(const mbx_dims &d, const mbx_pair_plan *pp) {
     FwdPlan p;
     const int na = d.na.local_size;
     const int nb = d.nbl.local_size;
+    p.pc = false;
+    p.cth = 0;
+    if (pp && pp->blocks && pp->nblocks > 0 && pp->producer_waves >= 0) {
+        // double-buffered producer / consumer kernel
+        p.blocked = true;
+        p.pc = true;
+        p.ni = 0;
+        const int maxth = pc_max_threads<T>();
+        int pw = pp->producer_waves;
+        const int max_cons = maxth - 64 * std::max(pw, 1);
+        p.nbchunk = (pp->nblocks + max_cons - 1) / max_cons;
+        p.bchunk = (pp->nblocks + p.nbchunk - 1) / p.nbchunk;
+        int ct = (256 + p.bchunk - 1) / p.bchunk;
+        ct = std::max(1, std::min(ct, 8));
+        while (ct > 1 && p.bchunk * ct > max_cons) --ct;
+        p.ct = std::min<int>(ct, d.nchan.local_size);
+        p.cth = ((p.bchunk * p.ct + 63) / 64) * 64;
+        // automatic: every wave the workgroup limit leaves, at most one
+        // producer per consumer wave (the beam gathers are latency bound)
+        if (pw == 0) pw = std::max(1, std::min((maxth - p.cth) / 64, p.cth / 64));
+        p.threads = p.cth + 64 * pw;
+        p.nctile = (d.nchan.local_size + p.ct - 1) / p.ct;
+        int st = 32;
+        while (st > 1 && pc_lds_bytes<T>(na, p.ct, st, p.threads / 64) > kLdsBudget) st >>= 1;
+        p.st = st;
+        p.lds_bytes = pc_lds_bytes<T>(na, p.ct, st, p.threads / 64);
+        p.nwg = uint64_t(d.ntime.local_size) * p.nctile * p.nbchunk;
+        return p;
+    }
     if (pp && pp->blocks && pp->nblocks > 0) {
         p.blocked = true;
         p.ni = 0;
@@ -391,6 +565,15 @@ template <typename T>
 static void launch_blocked(const FwdPlan &p, const mbx_pair_plan *pp, const RimeArgs<T> &A,
                            hipStream_t stream) {
     dim3 grid((unsigned)p.nwg), block(p.threads);
+    if (p.pc) {
+        if (pp->pb == 2 && pp->qb == 2)
+            hipLaunchKernelGGL((k_forward_pc<T, 2, 2>), grid, block, p.lds_bytes, stream, A, p.cth);
+        else if (pp->pb == 2 && pp->qb == 4)
+            hipLaunchKernelGGL((k_forward_pc<T, 2, 4>), grid, block, p.lds_bytes, stream, A, p.cth);
+        else
+            hipLaunchKernelGGL((k_forward_pc<T, 1, 1>), grid, block, p.lds_bytes, stream, A, p.cth);
+        return;
+    }
     if (pp->pb == 2 && pp->qb == 2)
         hipLaunchKernelGGL((k_forward_blk<T, 2, 2>), grid, block, p.lds_bytes, stream, A);
     else if (pp->pb == 2 && pp->qb == 4)

@@ -50,6 +50,7 @@ template <typename T> struct Tile {
     J4<T> *J;        // [st][NA][ct]
     double *part;    // [nwaves]
 
+    Tile() = default;
     __device__ __forceinline__ Tile(unsigned char *base, int NA, int ct, int st) {
         uvw = reinterpret_cast<double *>(base);
         pa = uvw + 3 * NA;
@@ -147,8 +148,20 @@ template <typename T> struct PhaseCS {
 };
 
 template <typename T>
+__device__ __forceinline__ void tile_setup(const RimeArgs<T> &A, double *Luvw, double *Lpa,
+                                           double *Lch, int t, int c0, int tid, int nth);
+
+template <typename T>
 __device__ __forceinline__ void tile_setup(const RimeArgs<T> &A, Tile<T> &L, int t, int c0,
                                            int tid, int nth) {
+    tile_setup<T>(A, L.uvw, L.pa, L.ch, t, c0, tid, nth);
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_setup(const RimeArgs<T> &A, double *Luvw, double *Lpa,
+                                           double *Lch, int t, int c0, int tid, int nth) {
+    Tile<T> L;
+    L.uvw = Luvw; L.pa = Lpa; L.ch = Lch;
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
     const int nc_ext = dext(d.nchan);
@@ -216,13 +229,14 @@ __device__ __forceinline__ void tile_phase0(const RimeArgs<T> &A, Tile<T> &L, in
 // (RimeEBeam.py:233-343 / CPUSolver.py:348-475 semantics); grid
 // coordinates in float64.
 template <typename T>
-__device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const Tile<T> &L, int t, int a,
-                                            int c, int k, double l0, double m0) {
+__device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const double *Lpa,
+                                            const double *Lch, int t, int a, int c, int k,
+                                            double l0, double m0) {
     const mbx_dims &d = A.d;
     const uint64_t NA = d.na.local_size, NC = d.nchan.local_size;
     const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
     const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
-    const double ps = L.pa[2 * a], pc = L.pa[2 * a + 1];
+    const double ps = Lpa[2 * a], pc = Lpa[2 * a + 1];
     double l = (l0 * pc - m0 * ps + double(pet[0])) * double(asc[0]);
     double m = (l0 * ps + m0 * pc + double(pet[1])) * double(asc[1]);
     const double LW1 = double(d.beam_lw.local_size - 1), MH1 = double(d.beam_mh.local_size - 1);
@@ -230,32 +244,30 @@ __device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const Tile<T> 
     double gl0 = floor(vl);
     double vm = fmin(MH1, fmax(0.0, A.mscale * (m - A.lmin)));
     double gm0 = floor(vm);
-    const T ld = T(vl - gl0), md = T(vm - gm0), chd = T(L.ch[CH * k + 2]);
+    const T ld = T(vl - gl0), md = T(vm - gm0), chd = T(Lch[CH * k + 2]);
     const uint32_t MH = d.beam_mh.local_size, NUD = d.beam_nud.local_size;
     const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
     const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const uint32_t ic0 = uint32_t(L.ch[CH * k]), ic1 = uint32_t(L.ch[CH * k + 1]);
+    const uint32_t ic0 = uint32_t(Lch[CH * k]), ic1 = uint32_t(Lch[CH * k + 1]);
     const T wc0 = T(1) - chd, wc1 = chd;
     cx<T> acc[4];
     T amp[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) { acc[p] = {T(0), T(0)}; amp[p] = T(0); }
-    // Four (l, m) corners, one at a time (bounds the loads in flight);
-    // both channel taps of a corner are adjacent in the cube.
+    // The eight taps one at a time: bounds the registers held by loads in
+    // flight (the kernels hide the latency with other waves).
 #pragma unroll 1
-    for (int corner = 0; corner < 4; ++corner) {
-        const bool hl = corner & 1, hm = corner >> 1;
-        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0;
-        const T wlm = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md);
-        const BeamTap<T> *row = A.beam + (uint64_t(il * MH + im) * NUD) * 4;
-        const BeamTap<T> *t0 = row + ic0 * 4, *t1 = row + ic1 * 4;
-        const T w0 = wlm * wc0, w1 = wlm * wc1;
+    for (int tap = 0; tap < 8; ++tap) {
+        const bool hl = tap & 1, hm = (tap >> 1) & 1, hc = tap >> 2;
+        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0, ic = hc ? ic1 : ic0;
+        const T w = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md) * (hc ? wc1 : wc0);
+        const BeamTap<T> *e = A.beam + ((uint64_t(il * MH + im) * NUD) + ic) * 4;
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            const BeamTap<T> e0 = t0[p], e1 = t1[p];
-            acc[p].re = fma(w1, e1.re, fma(w0, e0.re, acc[p].re));
-            acc[p].im = fma(w1, e1.im, fma(w0, e0.im, acc[p].im));
-            amp[p] = fma(w1, e1.amp, fma(w0, e0.amp, amp[p]));
+            const BeamTap<T> v = e[p];
+            acc[p].re = fma(w, v.re, acc[p].re);
+            acc[p].im = fma(w, v.im, acc[p].im);
+            amp[p] = fma(w, v.amp, amp[p]);
         }
     }
     J4<T> E;
@@ -283,7 +295,7 @@ __device__ __forceinline__ void tile_phase1(const RimeArgs<T> &A, Tile<T> &L, in
         const int si = i / (ct * NA);
         const int c = min(c0 + k, nc_ext - 1);
         const double l0 = L.src[3 * si], m0 = L.src[3 * si + 1], nm1 = L.src[3 * si + 2];
-        J4<T> E = beam_jones<T>(A, L, t, a, c, k, l0, m0);
+        J4<T> E = beam_jones<T>(A, L.pa, L.ch, t, a, c, k, l0, m0);
         const double *x = L.uvw + 3 * a;
         double p = x[2] * nm1 + x[1] * m0 + x[0] * l0;
         double phi = -A.two_pi_over_c * L.ch[CH * k + 3] * p;
@@ -295,6 +307,104 @@ __device__ __forceinline__ void tile_phase1(const RimeArgs<T> &A, Tile<T> &L, in
 #pragma unroll
         for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
         L.J[i] = jmul(E, KB);
+    }
+}
+
+template <typename T> __device__ __forceinline__ T exp_t(double x);
+template <> __device__ __forceinline__ float exp_t<float>(double x) { return expf(float(x)); }
+template <> __device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
+
+// Per-antenna Jones of one (source s, antenna a, channel c = c0 + k) with
+// every source quantity read from global memory (producer waves of the
+// double-buffered kernels, which share no per-source LDS stage).
+template <typename T>
+__device__ __forceinline__ J4<T> antenna_jones(const RimeArgs<T> &A, const double *Luvw,
+                                               const double *Lpa, const double *Lch, int t, int s,
+                                               int a, int c, int k) {
+    const uint64_t NT = A.d.ntime.local_size;
+    const double l0 = A.lm[2 * uint64_t(s)], m0 = A.lm[2 * uint64_t(s) + 1];
+    const double nm1 = sqrt(1.0 - l0 * l0 - m0 * m0) - 1.0;
+    const T *S = A.stokes + (uint64_t(s) * NT + t) * 4;
+    const T power = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * Lch[CH * k + 4]);
+    const J4<T> B = brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
+    const J4<T> E = beam_jones<T>(A, Lpa, Lch, t, a, c, k, l0, m0);
+    const double *x = Luvw + 3 * a;
+    const double phi = -A.two_pi_over_c * Lch[CH * k + 3] * (x[2] * nm1 + x[1] * m0 + x[0] * l0);
+    T sn, cs;
+    PhaseCS<T>::eval(phi, &sn, &cs);
+    const cx<T> K = {cs, sn};
+    J4<T> KB;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
+    return jmul(E, KB);
+}
+
+// Shape parameters of source s: p0, p1, p2 and type (0 point, 1 gaussian,
+// 2 sersic; for sersic p2 = R / (1 - e1^2 - e2^2)).
+template <typename T>
+__device__ __forceinline__ void source_shape(const RimeArgs<T> &A, int s, T *out) {
+    const mbx_dims &d = A.d;
+    const int np = dext(d.npsrc), ng = dext(d.ngsrc);
+    const uint64_t NG = d.ngsrc.local_size, NS = d.nssrc.local_size;
+    T p0 = T(0), p1 = T(0), p2 = T(0), ty = T(0);
+    if (s >= np && s < np + ng) {
+        const int g = s - np;
+        p0 = A.gauss[g]; p1 = A.gauss[g + NG]; p2 = A.gauss[g + 2 * NG]; ty = T(1);
+    } else if (s >= np + ng) {
+        const int g = s - np - ng;
+        p0 = A.sersic[g]; p1 = A.sersic[g + NS];
+        p2 = A.sersic[g + 2 * NS] / (T(1) - p0 * p0 - p1 * p1); ty = T(2);
+    }
+    out[0] = p0; out[1] = p1; out[2] = p2; out[3] = ty;
+}
+
+// LDS of the double-buffered (producer / consumer) kernels.
+template <typename T> struct PcTile {
+    double *uvw;   // [NA][3]
+    double *pa;    // [NA][2]
+    double *ch;    // [ct][CH]
+    T *shape;      // [2][st][4]
+    J4<T> *J;      // [2][st][NA][ct]
+    double *part;  // [nwaves]
+    int jstride;   // st * NA * ct
+
+    __device__ __forceinline__ PcTile(unsigned char *base, int NA, int ct, int st) {
+        uvw = reinterpret_cast<double *>(base);
+        pa = uvw + 3 * NA;
+        ch = pa + 2 * NA;
+        shape = reinterpret_cast<T *>(ch + CH * ct);
+        uintptr_t off = reinterpret_cast<uintptr_t>(shape + 2 * 4 * st);
+        off = (off + 15) & ~uintptr_t(15);
+        J = reinterpret_cast<J4<T> *>(off);
+        jstride = st * NA * ct;
+        part = reinterpret_cast<double *>(J + 2 * jstride);
+    }
+};
+
+template <typename T>
+static inline size_t pc_lds_bytes(int na, int ct, int st, int nwaves) {
+    return sizeof(double) * (5 * size_t(na) + CH * ct) + sizeof(T) * 8 * st + 16 +
+           sizeof(J4<T>) * 2 * size_t(st) * na * ct + sizeof(double) * nwaves;
+}
+
+// Producer step: fill buffer `buf` with tile [s0, s0 + nst) using the
+// producer threads ptid in [0, pth).
+template <typename T>
+__device__ __forceinline__ void pc_produce(const RimeArgs<T> &A, PcTile<T> &L, int buf, int t,
+                                           int c0, int s0, int nst, int ptid, int pth) {
+    const int NA = A.d.na.local_size;
+    const int ct = A.ct;
+    const int nc_ext = dext(A.d.nchan);
+    J4<T> *J = L.J + buf * L.jstride;
+    T *shape = L.shape + buf * 4 * A.st;
+    for (int i = ptid; i < nst; i += pth) source_shape<T>(A, s0 + i, shape + 4 * i);
+    const int n1 = nst * NA * ct;
+    for (int i = ptid; i < n1; i += pth) {
+        const int k = i % ct;
+        const int a = (i / ct) % NA;
+        const int si = i / (ct * NA);
+        const int c = min(c0 + k, nc_ext - 1);
+        J[i] = antenna_jones<T>(A, L.uvw, L.pa, L.ch, t, s0 + si, a, c, k);
     }
 }
 

@@ -144,21 +144,29 @@ def alloc_outputs(arrays, keep_ekb=False):
     return out
 
 
-# Default pair-block shapes: 2x2 antenna pairs per thread in float; in
-# double the accumulators are twice as large and 1x1 keeps them in registers.
-DEFAULT_PAIR_BLOCK = {'f32': (2, 2), 'f64': (2, 2)}
+# Default kernel form per dtype: (pb, qb, producer_waves) -- pair blocks of
+# pb x qb antenna pairs per consumer thread; producer_waves 0 = the
+# double-buffered kernel with an automatic producer count, < 0 = the
+# single-role pair-block kernel; 'generic' = per-baseline kernel (measured
+# fastest in double, where the pair-block accumulators spill).
+DEFAULT_PAIR_BLOCK = {'f32': (2, 2, 0), 'f64': 'generic'}
 
 
 def pair_plan(arrays, pair_block=None):
     """Device pair-block plan for the antenna layout of ``arrays`` (cached in
     the dict under '_pair_plan'); None selects the generic kernel.
 
-    pair_block: (pb, qb), 'generic', or None for the dtype default.
+    pair_block: (pb, qb[, producer_waves]), 'generic', or None for the
+    dtype default.
     """
+    if pair_block is None:
+        pair_block = DEFAULT_PAIR_BLOCK[suffix(arrays)]
     if pair_block == 'generic':
         return None
-    pb, qb = DEFAULT_PAIR_BLOCK[suffix(arrays)] if pair_block is None else pair_block
-    key = ('_pair_plan', pb, qb)
+    pbq = tuple(pair_block)
+    pb, qb = pbq[0], pbq[1]
+    pw = pbq[2] if len(pbq) > 2 else 0
+    key = ('_pair_plan', pb, qb, pw)
     cached = arrays.get('_pair_plan')
     if cached is not None and cached[0] == key:
         return cached[1]
@@ -168,7 +176,7 @@ def pair_plan(arrays, pair_block=None):
     plan = None
     if host is not None:
         dev = torch.from_numpy(host).to(arrays['uvw'].device)
-        plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, 0), dev)
+        plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, pw), dev)
     arrays['_pair_plan'] = (key, plan)
     return plan
 

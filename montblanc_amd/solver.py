@@ -353,7 +353,11 @@ class RimeSolver(object):
         antenna1/antenna2 transfers); None when the layout varies with time."""
         if self._pair_plan is None:
             from .rime import DEFAULT_PAIR_BLOCK
-            pb, qb = DEFAULT_PAIR_BLOCK['f32' if self.is_float() else 'f64']
+            form = DEFAULT_PAIR_BLOCK['f32' if self.is_float() else 'f64']
+            if form == 'generic':
+                self._pair_plan = False
+                return None
+            pb, qb, pw = form
             for i, name in enumerate(('antenna1', 'antenna2')):
                 if self._host_antenna[i] is None:
                     self._host_antenna[i] = self.retrieve(name)
@@ -362,7 +366,7 @@ class RimeSolver(object):
             plan = None
             if host is not None:
                 dev = torch.from_numpy(host).to(self.device)
-                plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, 0), dev)
+                plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, pw), dev)
             self._pair_plan = plan if plan is not None else False
         return self._pair_plan or None
 

@@ -158,7 +158,8 @@ def test_forward_rejects_bad_arguments():
         rime.forward(bad, (1, 0, 0))
 
 
-VARIANTS = ['generic', (1, 1), (2, 2), (2, 4)]
+VARIANTS = ['generic', (1, 1, -1), (2, 2, -1), (2, 4, -1), (1, 1, 0), (2, 2, 0), (2, 2, 3),
+            (2, 4, 1)]
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
