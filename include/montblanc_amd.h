@@ -215,6 +215,48 @@ int mbx_forward_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *
                     const mbx_pair_plan *plan, int options,
                     void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- Adjoint: d chi^2 / d(parameters), recomputing the Jones terms ----
+ * No reference counterpart (the reference has no gradient). */
+
+/* Gradient outputs, float64 whatever the input dtype (accumulated across
+ * workgroups). Real parameters get d chi^2/d theta; the complex gains get
+ * d chi^2/dRe + i d chi^2/dIm (torch's convention for complex tensors).
+ * All are multiplied by grad_scale (the upstream d L / d chi^2). Any
+ * pointer may be NULL to skip that gradient. */
+typedef struct mbx_grads {
+    double *lm;            /* (nsrc, 2)                      */
+    double *stokes;        /* (nsrc, ntime, 4)               */
+    double *alpha;         /* (nsrc, ntime)                  */
+    double *gauss_shape;   /* (3, ngsrc)                     */
+    double *sersic_shape;  /* (3, nssrc)                     */
+    void *G_term;          /* (ntime, na, nchan, 4) complex128 */
+} mbx_grads;
+
+/* Host helper: (na x na) int32 map from an antenna pair to its baseline:
+ * out[p*na + q] = b for baseline b = (p, q) with p = antenna1,
+ * b | MBX_BL_REVERSED at [q*na + p], b | MBX_BL_AUTO for p == q, -1 where
+ * no baseline. Fails (MBX_E_UNSUPPORTED) when the layout changes with
+ * time or a pair occurs twice: the adjoint needs one baseline per pair. */
+#define MBX_BL_REVERSED (1 << 30)
+#define MBX_BL_AUTO (1 << 29)
+int mbx_build_baseline_map(const int32_t *antenna1, const int32_t *antenna2, int32_t ntime,
+                           int32_t nbl, int32_t na, int32_t *out);
+
+/* Backward pass after a forward over the same inputs: ekb_vis is the
+ * forward's pre-gain source sum (mbx_outputs.ekb_vis), baseline_map a
+ * DEVICE copy of mbx_build_baseline_map's output. The source extents must
+ * cover every source (lower 0, upper global): the adjoint needs the full
+ * model. */
+size_t mbx_adjoint_workspace_bytes(const mbx_dims *d, int is_double);
+int mbx_adjoint_f32(const mbx_dims *d, const mbx_inputs *in, const void *ekb_vis,
+                    const int32_t *baseline_map, const mbx_grads *grads,
+                    const mbx_beam_extents *beam, const mbx_consts *k, int options,
+                    double grad_scale, void *workspace, size_t workspace_bytes, void *stream);
+int mbx_adjoint_f64(const mbx_dims *d, const mbx_inputs *in, const void *ekb_vis,
+                    const int32_t *baseline_map, const mbx_grads *grads,
+                    const mbx_beam_extents *beam, const mbx_consts *k, int options,
+                    double grad_scale, void *workspace, size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

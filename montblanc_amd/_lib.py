@@ -75,6 +75,16 @@ class mbx_pair_plan(ctypes.Structure):
                 ('producer_waves', ctypes.c_int32)]
 
 
+GRAD_NAMES = ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape', 'G_term')
+
+
+class mbx_grads(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in GRAD_NAMES]
+
+
+BL_REVERSED = 1 << 30
+BL_AUTO = 1 << 29
+
 _P = ctypes.c_void_p
 _PD = ctypes.POINTER(mbx_dims)
 _PB = ctypes.POINTER(mbx_beam_extents)
@@ -95,6 +105,8 @@ SIGNATURES = {
     'mbx_forward_workspace_bytes': ([_PD, _I], _SZ),
     'mbx_pair_plan_capacity': ([_I32, _I32, _I32], _I64),
     'mbx_build_pair_plan': ([_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64], _I64),
+    'mbx_build_baseline_map': ([_P, _P, _I32, _I32, _I32, _P], _I),
+    'mbx_adjoint_workspace_bytes': ([_PD, _I], _SZ),
 }
 for _t in ('f32', 'f64'):
     SIGNATURES.update({
@@ -104,6 +116,8 @@ for _t in ('f32', 'f64'):
         'mbx_sum_coherencies_' + _t: ([_PD] + [_P] * 13 + [_PK, _I, _P], _I),
         'mbx_chi2_reduce_' + _t: ([_PD, _P, _P, _PK, _I, _P, _SZ, _P], _I),
         'mbx_forward_' + _t: ([_PD, _PI, _PO, _PB, _PK, _PP, _I, _P, _SZ, _P], _I),
+        'mbx_adjoint_' + _t: ([_PD, _PI, _P, _P, ctypes.POINTER(mbx_grads), _PB, _PK, _I,
+                               ctypes.c_double, _P, _SZ, _P], _I),
     })
 
 _lib = None
@@ -175,3 +189,14 @@ def build_pair_plan(antenna1, antenna2, na, pb, qb):
     if n < 0:
         raise MontblancHipError(int(n), 'mbx_build_pair_plan', lib.mbx_last_error().decode())
     return out[: n * (2 + pb * qb)].reshape(n, 2 + pb * qb)
+
+
+def build_baseline_map(antenna1, antenna2, na):
+    """(na, na) int32 pair -> baseline map for the adjoint (host array)."""
+    import numpy as np
+    a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
+    a2 = np.ascontiguousarray(antenna2, dtype=np.int32)
+    ntime, nbl = a1.shape
+    out = np.empty((na, na), dtype=np.int32)
+    call('mbx_build_baseline_map', a1.ctypes.data, a2.ctypes.data, ntime, nbl, na, out.ctypes.data)
+    return out

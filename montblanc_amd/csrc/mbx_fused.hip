@@ -30,16 +30,6 @@ namespace mbx {
 
 void launch_chi2_final(const double *partial, unsigned n, double scale, double *X2, hipStream_t st);
 
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_beam_prepare(const cx<T> *__restrict__ E, BeamTap<T> *__restrict__ out, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        cx<T> e = E[i];
-        out[i] = {e.re, e.im, sqrt(e.re * e.re + e.im * e.im), T(0)};
-    }
-}
-
 constexpr int kGenThreads = 256;
 
 // Workgroup index -> (baseline chunk, channel tile, time).
@@ -505,28 +495,6 @@ static size_t forward_ws_bytes(const mbx_dims &d) {
     return beam_ws_bytes<T>(d) + max_partials(d) * sizeof(double);
 }
 
-template <typename T>
-static void fill_args(RimeArgs<T> &A, const mbx_dims *d, const mbx_inputs *in,
-                      const mbx_beam_extents *b, const mbx_consts *k, int options) {
-    A.d = *d;
-    A.uvw = (const T *)in->uvw; A.lm = (const T *)in->lm; A.stokes = (const T *)in->stokes;
-    A.alpha = (const T *)in->alpha; A.freq = (const T *)in->frequency;
-    A.ref_freq = (const T *)in->ref_frequency; A.pa = (const T *)in->parallactic_angles;
-    A.pe = (const T *)in->point_errors; A.ascale = (const T *)in->antenna_scaling;
-    A.gauss = (const T *)in->gauss_shape; A.sersic = (const T *)in->sersic_shape;
-    A.wv = (const T *)in->weight_vector;
-    A.a1 = in->antenna1; A.a2 = in->antenna2; A.flag = in->flag;
-    A.obs = (const cx<T> *)in->observed_vis; A.G = (const cx<T> *)in->G_term;
-    A.ll = b->ll; A.lmin = b->lm;
-    A.lscale = double(d->beam_lw.local_size - 1) / (b->ul - b->ll);
-    A.mscale = double(d->beam_mh.local_size - 1) / (b->um - b->lm);
-    A.lfreq = b->lfreq;
-    A.fscale = double(d->beam_nud.local_size - 1) / (b->ufreq - b->lfreq);
-    A.gauss_scale = k->gauss_scale; A.two_pi_over_c = k->two_pi_over_c;
-    A.options = options;
-    A.blocks = nullptr; A.nblocks = 0; A.pb = A.qb = 0;
-}
-
 int validate_inputs(const mbx_dims *d, const mbx_inputs *in, const mbx_beam_extents *b,
                     const mbx_consts *k, int options, const char *who) {
     int r = check_dims(d, true);
@@ -609,9 +577,7 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
         if (final_batch) launch_chi2_final(partial, 0, scale, out->X2, stream);
         return hip_status(hipGetLastError(), "forward (empty extents)");
     }
-    unsigned g = (unsigned)std::min<uint64_t>((ncube + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_beam_prepare<T>, dim3(g), dim3(256), 0, stream,
-                       static_cast<const cx<T> *>(in->E_beam), beam, ncube);
+    launch_beam_prepare<T>(static_cast<const cx<T> *>(in->E_beam), beam, ncube, stream);
 
     RimeArgs<T> A;
     fill_args(A, d, in, b, k, options);

@@ -61,4 +61,33 @@ int64_t mbx_build_pair_plan(const int32_t *a1, const int32_t *a2, int32_t ntime,
     return n;
 }
 
+int mbx_build_baseline_map(const int32_t *a1, const int32_t *a2, int32_t ntime, int32_t nbl,
+                           int32_t na, int32_t *out) {
+    using mbx::set_error;
+    if (!a1 || !a2 || !out) return set_error(MBX_E_NULL, "build_baseline_map: NULL pointer");
+    if (ntime <= 0 || nbl <= 0 || na <= 0) return set_error(MBX_E_DIMS, "build_baseline_map: bad sizes");
+    for (int64_t t = 1; t < ntime; ++t)
+        for (int64_t b = 0; b < nbl; ++b)
+            if (a1[t * nbl + b] != a1[b] || a2[t * nbl + b] != a2[b])
+                return set_error(MBX_E_UNSUPPORTED,
+                                 "build_baseline_map: antenna pairs change at time %lld", (long long)t);
+    for (int64_t i = 0; i < int64_t(na) * na; ++i) out[i] = -1;
+    for (int32_t b = 0; b < nbl; ++b) {
+        const int p = a1[b], q = a2[b];
+        if (p < 0 || p >= na || q < 0 || q >= na)
+            return set_error(MBX_E_DIMS, "build_baseline_map: baseline %d pairs (%d, %d) outside "
+                             "[0, %d)", b, p, q, na);
+        if (out[int64_t(p) * na + q] != -1 || out[int64_t(q) * na + p] != -1)
+            return set_error(MBX_E_UNSUPPORTED, "build_baseline_map: antennas (%d, %d) pair "
+                             "more than one baseline", p, q);
+        if (p == q) {
+            out[int64_t(p) * na + p] = b | MBX_BL_AUTO;
+        } else {
+            out[int64_t(p) * na + q] = b;
+            out[int64_t(q) * na + p] = b | MBX_BL_REVERSED;
+        }
+    }
+    return MBX_OK;
+}
+
 }  // extern "C"

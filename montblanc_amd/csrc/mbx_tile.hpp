@@ -37,6 +37,50 @@ template <typename T> struct RimeArgs {
     int nblocks, pb, qb;
 };
 
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_beam_prepare(const cx<T> *__restrict__ E, BeamTap<T> *__restrict__ out, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        cx<T> e = E[i];
+        out[i] = {e.re, e.im, sqrt(e.re * e.re + e.im * e.im), T(0)};
+    }
+}
+
+// Value and amplitude of every beam-cube entry, once per launch.
+template <typename T>
+static inline void launch_beam_prepare(const cx<T> *E, BeamTap<T> *beam, uint64_t n,
+                                       hipStream_t stream) {
+    unsigned g = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(k_beam_prepare<T>, dim3(g), dim3(256), 0, stream, E, beam, n);
+}
+
+// Argument block from the C-ABI structs (beam cube pointer, outputs and
+// tiling are set by the caller).
+template <typename T>
+static inline void fill_args(RimeArgs<T> &A, const mbx_dims *d, const mbx_inputs *in,
+                             const mbx_beam_extents *b, const mbx_consts *k, int options) {
+    A.d = *d;
+    A.uvw = (const T *)in->uvw; A.lm = (const T *)in->lm; A.stokes = (const T *)in->stokes;
+    A.alpha = (const T *)in->alpha; A.freq = (const T *)in->frequency;
+    A.ref_freq = (const T *)in->ref_frequency; A.pa = (const T *)in->parallactic_angles;
+    A.pe = (const T *)in->point_errors; A.ascale = (const T *)in->antenna_scaling;
+    A.gauss = (const T *)in->gauss_shape; A.sersic = (const T *)in->sersic_shape;
+    A.wv = (const T *)in->weight_vector;
+    A.a1 = in->antenna1; A.a2 = in->antenna2; A.flag = in->flag;
+    A.obs = (const cx<T> *)in->observed_vis; A.G = (const cx<T> *)in->G_term;
+    A.ll = b->ll; A.lmin = b->lm;
+    A.lscale = double(d->beam_lw.local_size - 1) / (b->ul - b->ll);
+    A.mscale = double(d->beam_mh.local_size - 1) / (b->um - b->lm);
+    A.lfreq = b->lfreq;
+    A.fscale = double(d->beam_nud.local_size - 1) / (b->ufreq - b->lfreq);
+    A.gauss_scale = k->gauss_scale; A.two_pi_over_c = k->two_pi_over_c;
+    A.options = options;
+    A.model = nullptr; A.chi2 = nullptr; A.ekb = nullptr; A.partial = nullptr;
+    A.blocks = nullptr; A.nblocks = 0; A.pb = A.qb = 0;
+    A.ct = A.st = A.nbchunk = A.bchunk = A.nctile = 1;
+}
+
 // LDS carve-up of one tile.
 constexpr int CH = 5;   // doubles per channel in the tile
 

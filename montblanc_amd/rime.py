@@ -205,6 +205,55 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
     return out
 
 
+def baseline_map(arrays):
+    """Device (na, na) pair -> baseline map of the antenna layout (cached in
+    the dict under '_baseline_map')."""
+    m = arrays.get('_baseline_map')
+    if m is None:
+        host = _lib.build_baseline_map(arrays['antenna1'].cpu().numpy(),
+                                       arrays['antenna2'].cpu().numpy(), arrays['uvw'].shape[1])
+        m = torch.from_numpy(host).to(arrays['uvw'].device)
+        arrays['_baseline_map'] = m
+    return m
+
+
+def alloc_grads(arrays, counts):
+    """float64 gradient buffers shaped like the parameters (G complex128)."""
+    dev = arrays['uvw'].device
+    nsrc = sum(counts)
+    ntime, na = arrays['uvw'].shape[:2]
+    nchan = arrays['frequency'].shape[0]
+    f64 = dict(dtype=torch.float64, device=dev)
+    return dict(lm=torch.empty((nsrc, 2), **f64), stokes=torch.empty((nsrc, ntime, 4), **f64),
+                alpha=torch.empty((nsrc, ntime), **f64),
+                gauss_shape=torch.empty((3, counts[1]), **f64),
+                sersic_shape=torch.empty((3, counts[2]), **f64),
+                G_term=torch.empty((ntime, na, nchan, 4), dtype=torch.complex128, device=dev))
+
+
+def adjoint(arrays, counts, ekb_vis, grad_scale=1.0, use_weight_vector=False, sigma_sqrd=None,
+            grads=None, stream=None):
+    """d chi^2 / d(lm, stokes, alpha, gauss_shape, sersic_shape, G_term) times
+    grad_scale, from the forward's pre-gain sums ``ekb_vis``. Returns a dict
+    of float64 tensors (G_term complex128: d/dRe + i d/dIm)."""
+    dev = arrays['uvw'].device
+    grads = alloc_grads(arrays, counts) if grads is None else grads
+    d = make_dims(arrays, counts)
+    bmap = baseline_map(arrays)
+    ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
+    g = _lib.mbx_grads(**{n: _raw(grads.get(n)) for n in _lib.GRAD_NAMES})
+    lib = _lib.load()
+    nb = lib.mbx_adjoint_workspace_bytes(ctypes.byref(d), int(is_double(arrays)))
+    ws = _workspace(dev).get('adjoint', nb)
+    b = beam_extents(arrays)
+    k = consts(arrays, sigma_sqrd)
+    h = stream_of(dev) if stream is None else ctypes.c_void_p(stream)
+    _lib.call('mbx_adjoint_' + suffix(arrays), ctypes.byref(d), ctypes.byref(ins), _ptr(ekb_vis),
+              _ptr(bmap), ctypes.byref(g), ctypes.byref(b), ctypes.byref(k),
+              options(False, use_weight_vector), float(grad_scale), _ptr(ws), nb, h)
+    return grads
+
+
 def forward_numpy(a, counts, device='cuda:0', **kw):
     """numpy in, numpy out (model_vis, chi_sqrd_result, X2 float)."""
     arrays = to_device(a, device)

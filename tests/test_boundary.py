@@ -233,3 +233,25 @@ def test_pair_plan_duplicates_time_variation_and_bad_indices():
     assert _lib.build_pair_plan(varying, varying + 1, 3, 2, 2) is None
     with pytest.raises(_lib.MontblancHipError):
         _lib.build_pair_plan(a1, a2 + 5, 3, 2, 2)
+
+
+def test_baseline_map_codes():
+    from montblanc_amd import _lib
+    from oracle import rime_np
+    na = 5
+    a1, a2 = rime_np.default_ant_pairs(na, 2, auto_correlations=True)
+    m = _lib.build_baseline_map(a1, a2, na)
+    for b in range(a1.shape[1]):
+        p, q = a1[0, b], a2[0, b]
+        if p == q:
+            assert m[p, p] == b | _lib.BL_AUTO
+        else:
+            assert m[p, q] == b and m[q, p] == b | _lib.BL_REVERSED
+    assert (m >= 0).all()
+    a1, a2 = rime_np.default_ant_pairs(na, 2)
+    m = _lib.build_baseline_map(a1, a2, na)
+    assert (np.diag(m) == -1).all()
+    varying = a1.copy()
+    varying[1] = varying[1][::-1]
+    with pytest.raises(_lib.MontblancHipError):
+        _lib.build_baseline_map(varying, a2, na)
