@@ -38,6 +38,7 @@ def parse():
     p.add_argument('--workload', default='forward_64ant')
     p.add_argument('--dtypes', default='f32,f64')
     p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-gradient', action='store_true')
     return p.parse_args()
 
 
@@ -95,6 +96,55 @@ def time_forward(w, dtype, steps, warmup):
                 terms=n_terms, flops=flops, X2=x2,
                 terms_per_s=n_terms / (wall / steps),
                 tflops=flops / (ms_mean * 1e-3) / 1e12)
+
+
+def time_gradient(w, dtype, steps, warmup):
+    """chi^2 alone vs chi^2 + full gradient (forward keeping the pre-gain
+    sums, then the adjoint) on the gradient workload."""
+    import numpy as np
+    import torch
+    from montblanc_amd import rime
+    from oracle import synth
+
+    counts = (w['npsrc'], w['ngsrc'], w['nssrc'])
+    a = synth.make_inputs(**w, seed=1002)
+    arrays = rime.to_device(synth.as_dtype(a, np.float32 if dtype == 'f32' else np.float64))
+    del a
+    out = rime.alloc_outputs(arrays, keep_ekb=True)
+    rime.forward(arrays, counts, outputs=out)
+    g = torch.Generator(device='cuda').manual_seed(8)
+    noise = torch.randn(out['model_vis'].shape + (2,), generator=g, device='cuda',
+                        dtype=arrays['uvw'].dtype) * 0.1
+    arrays['observed_vis'] = (out['model_vis'] + torch.view_as_complex(noise)).contiguous()
+    del noise
+    grads = rime.alloc_grads(arrays, counts)
+
+    def fwd():
+        rime.forward(arrays, counts, outputs=out)
+
+    def fwd_grad():
+        rime.forward(arrays, counts, outputs=out)
+        rime.adjoint(arrays, counts, out['ekb_vis'], grads=grads)
+
+    res = {}
+    for name, fn in (('forward', fwd), ('forward_gradient', fwd_grad)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) * 1e3 / steps
+    nbl = w['na'] * (w['na'] - 1) // 2
+    n_terms = sum(counts) * w['ntime'] * nbl * w['nchan']
+    del arrays, out, grads
+    torch.cuda.empty_cache()
+    return {'workload': 'gradient_27ant', 'dtype': dtype, 'terms': n_terms,
+            'forward_ms': res['forward'], 'forward_gradient_ms': res['forward_gradient'],
+            'forward_terms_per_s': n_terms / (res['forward'] * 1e-3),
+            'gradient_terms_per_s': n_terms / (res['forward_gradient'] * 1e-3),
+            'gradient_over_forward': res['forward'] / res['forward_gradient']}
 
 
 def cpu_baseline(w):
@@ -165,6 +215,13 @@ def main():
                           'frac_of_peak': v['tflops'] / roofline.PEAK_TFLOPS[k], 'X2': v['X2']}
                       for k, v in results.items()},
     }
+    if not args.no_gradient:
+        gw = {k: v for k, v in synth.WORKLOADS['gradient_27ant'].items()}
+        grad = time_gradient(gw, 'f64', max(3, args.steps // 2), 1)
+        print('# gradient f64: forward %.2f ms, forward+gradient %.2f ms, ratio %.3f'
+              % (grad['forward_ms'], grad['forward_gradient_ms'], grad['gradient_over_forward']),
+              file=sys.stderr, flush=True)
+        line['gradient'] = grad
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(w)
     print(json.dumps(line), flush=True)

@@ -257,6 +257,50 @@ int mbx_adjoint_f64(const mbx_dims *d, const mbx_inputs *in, const void *ekb_vis
                     const mbx_beam_extents *beam, const mbx_consts *k, int options,
                     double grad_scale, void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- Batched evaluation of many sky-model samples ---------------------
+ * No reference counterpart. K samples, each with its own sources, against
+ * ONE shared set of visibility-space inputs (uvw, antennas, frequencies,
+ * beam, gains, flags, weights, observed visibilities) in one persistent
+ * launch. The source arrays of all samples are concatenated in the
+ * mbx_inputs (lm, stokes, alpha: rows; gauss_shape, sersic_shape: columns)
+ * and mbx_dims' source sizes are the totals. All arrays below are DEVICE
+ * int32 arrays of length nsamples (src_offset: nsamples + 1). */
+typedef struct mbx_batch {
+    int32_t nsamples;
+    const int32_t *src_offset;     /* first row of sample k in lm/stokes/alpha  */
+    const int32_t *npsrc, *ngsrc, *nssrc;   /* per-sample type counts          */
+    const int32_t *gauss_offset;   /* first gauss_shape column of sample k     */
+    const int32_t *sersic_offset;  /* first sersic_shape column of sample k    */
+    const int32_t *order;          /* processing order of the samples (a
+                                      permutation; largest first balances)    */
+} mbx_batch;
+
+/* Launch shape; zero fields take the defaults (also what the autotuner
+ * in montblanc_amd/autotune.py varies). */
+typedef struct mbx_batch_tuning {
+    int32_t workgroups;          /* persistent workgroups (default 2 per CU)  */
+    int32_t channels_per_item;   /* channels per work item (default 1)        */
+    int32_t sources_per_tile;    /* forward source tile (default: LDS bound)  */
+    int32_t reserved;
+} mbx_batch_tuning;
+
+/* X2[k] = chi^2 of sample k (device float64 [nsamples]). With grads
+ * (lm/stokes/alpha/gauss_shape/sersic_shape shaped like the concatenated
+ * inputs, float64; G_term must be NULL) the parameter gradients of every
+ * sample's chi^2 are produced in the same launch; baseline_map as for
+ * mbx_adjoint_*. Model visibilities are not written. */
+size_t mbx_batched_workspace_bytes(const mbx_dims *d, int is_double, const mbx_batch_tuning *tune);
+int mbx_batched_f32(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *batch, double *X2,
+                    const mbx_grads *grads, const int32_t *baseline_map,
+                    const mbx_beam_extents *beam, const mbx_consts *k,
+                    const mbx_batch_tuning *tune, int options, void *workspace,
+                    size_t workspace_bytes, void *stream);
+int mbx_batched_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *batch, double *X2,
+                    const mbx_grads *grads, const int32_t *baseline_map,
+                    const mbx_beam_extents *beam, const mbx_consts *k,
+                    const mbx_batch_tuning *tune, int options, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

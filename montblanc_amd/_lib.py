@@ -82,6 +82,17 @@ class mbx_grads(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in GRAD_NAMES]
 
 
+class mbx_batch(ctypes.Structure):
+    _fields_ = [('nsamples', ctypes.c_int32)] + [
+        (n, ctypes.c_void_p) for n in ('src_offset', 'npsrc', 'ngsrc', 'nssrc', 'gauss_offset',
+                                       'sersic_offset', 'order')]
+
+
+class mbx_batch_tuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ('workgroups', 'channels_per_item',
+                                              'sources_per_tile', 'reserved')]
+
+
 BL_REVERSED = 1 << 30
 BL_AUTO = 1 << 29
 
@@ -107,6 +118,7 @@ SIGNATURES = {
     'mbx_build_pair_plan': ([_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64], _I64),
     'mbx_build_baseline_map': ([_P, _P, _I32, _I32, _I32, _P], _I),
     'mbx_adjoint_workspace_bytes': ([_PD, _I], _SZ),
+    'mbx_batched_workspace_bytes': ([_PD, _I, ctypes.POINTER(mbx_batch_tuning)], _SZ),
 }
 for _t in ('f32', 'f64'):
     SIGNATURES.update({
@@ -118,6 +130,8 @@ for _t in ('f32', 'f64'):
         'mbx_forward_' + _t: ([_PD, _PI, _PO, _PB, _PK, _PP, _I, _P, _SZ, _P], _I),
         'mbx_adjoint_' + _t: ([_PD, _PI, _P, _P, ctypes.POINTER(mbx_grads), _PB, _PK, _I,
                                ctypes.c_double, _P, _SZ, _P], _I),
+        'mbx_batched_' + _t: ([_PD, _PI, ctypes.POINTER(mbx_batch), _P, ctypes.POINTER(mbx_grads),
+                               _P, _PB, _PK, ctypes.POINTER(mbx_batch_tuning), _I, _P, _SZ, _P], _I),
     })
 
 _lib = None

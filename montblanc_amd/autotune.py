@@ -1,0 +1,138 @@
+"""Project-written autotuner for launch shapes.
+
+Candidates are timed on the device (median of a few runs, synchronised)
+on a probe of the real problem -- the same arrays with the time extent cut
+to a few steps through the dims struct, so the per-item work is the real
+one -- and the fastest is cached per (kernel, dtype, shape, device) in
+memory and in a JSON file ($MBX_AUTOTUNE_CACHE, default
+~/.cache/montblanc_amd/autotune.json). Cached choices are reused without
+timing, so tuned calls are safe inside HIP-graph capture once warmed up.
+"""
+
+import ctypes
+import json
+import os
+import time
+
+import torch
+
+from . import _lib, rime
+
+_MEM = {}
+
+
+def cache_path():
+    return os.environ.get('MBX_AUTOTUNE_CACHE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'montblanc_amd',
+                                       'autotune.json'))
+
+
+def _load_disk():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_disk(key, value):
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = _load_disk()
+        data[key] = value
+        tmp = path + '.tmp%d' % os.getpid()
+        with open(tmp, 'w') as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass      # a read-only home only loses the persistence
+
+
+def _device_name(device):
+    return torch.cuda.get_device_name(device).replace(' ', '_')
+
+
+def time_call(fn, repeats=3):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def tune(key, candidates, run, use_cache=True):
+    """Return the candidate with the smallest run(candidate) time; failing
+    candidates (MontblancHipError) are skipped."""
+    if use_cache:
+        if key in _MEM:
+            return _MEM[key]
+        disk = _load_disk()
+        if key in disk:
+            _MEM[key] = disk[key]['choice']
+            return _MEM[key]
+    best, best_t, times = None, None, {}
+    for c in candidates:
+        try:
+            t = time_call(lambda: run(c))
+        except _lib.MontblancHipError:
+            continue
+        times[json.dumps(c)] = t
+        if best_t is None or t < best_t:
+            best, best_t = c, t
+    if best is None:
+        raise RuntimeError('autotune %s: no candidate ran' % key)
+    _MEM[key] = best
+    _store_disk(key, {'choice': best, 'seconds': times})
+    return best
+
+
+def _probe_dims(arrays, counts, ntime_probe):
+    d = rime.make_dims(arrays, counts)
+    d.ntime.upper_extent = min(d.ntime.local_size, ntime_probe)
+    return d
+
+
+def batched_tuning(batch, gradients=False, ntime_probe=2):
+    """Launch shape of the batched kernel for this batch's shape."""
+    from . import batched as bm
+    a = batch.arrays
+    na = a['uvw'].shape[1]
+    key = 'batched|%s|na%d|nbl%d|nchan%d|K%d|grad%d|%s' % (
+        rime.suffix(a), na, a['antenna1'].shape[1], a['frequency'].shape[0], batch.nsamples,
+        int(gradients), _device_name(batch.device))
+    props = torch.cuda.get_device_properties(batch.device)
+    cus = props.multi_processor_count
+    cands = []
+    for wg in (cus, 2 * cus, 4 * cus):
+        for cpi in (1, 2):
+            for st in (0, 8):
+                cands.append({'workgroups': wg, 'channels_per_item': cpi, 'sources_per_tile': st})
+
+    probe = _probe_dims(a, batch.totals, ntime_probe)
+
+    def run(c):
+        bm.evaluate(batch, gradients=gradients, tune=c, dims=probe)
+    return tune(key, cands, run)
+
+
+def forward_tuning(arrays, counts, ntime_probe=4):
+    """Kernel form (pair blocks / producer waves / generic) of the fused
+    forward for this shape."""
+    a = arrays
+    key = 'forward|%s|na%d|nbl%d|nchan%d|src%d|%s' % (
+        rime.suffix(a), a['uvw'].shape[1], a['antenna1'].shape[1], a['frequency'].shape[0],
+        sum(counts), _device_name(a['uvw'].device))
+    cands = ['generic', [2, 2, -1], [2, 2, 0], [2, 2, 3], [2, 2, 5], [1, 1, 0]]
+    d = _probe_dims(a, counts, ntime_probe)
+
+    def run(c):
+        a.pop('_pair_plan', None)
+        rime.forward(a, counts, dims=d, pair_block=c)
+    choice = tune(key, cands, run)
+    a.pop('_pair_plan', None)
+    return choice if choice == 'generic' else tuple(choice)

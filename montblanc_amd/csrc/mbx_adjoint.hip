@@ -22,36 +22,9 @@
 
 #include <algorithm>
 
-#include "mbx_tile.hpp"
+#include "mbx_backward.hpp"
 
 namespace mbx {
-
-template <typename T>
-__device__ __forceinline__ J4<T> jherm(const J4<T> &A) {
-    J4<T> r;
-    r.v[0] = {A.v[0].re, -A.v[0].im};
-    r.v[1] = {A.v[2].re, -A.v[2].im};
-    r.v[2] = {A.v[1].re, -A.v[1].im};
-    r.v[3] = {A.v[3].re, -A.v[3].im};
-    return r;
-}
-
-template <typename T>
-__device__ __forceinline__ J4<T> jadd(const J4<T> &A, const J4<T> &B) {
-    J4<T> r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r.v[i] = {A.v[i].re + B.v[i].re, A.v[i].im + B.v[i].im};
-    return r;
-}
-
-// Re <A, B>
-template <typename T>
-__device__ __forceinline__ T jdot_re(const J4<T> &A, const J4<T> &B) {
-    T s = T(0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) s += A.v[i].re * B.v[i].re + A.v[i].im * B.v[i].im;
-    return s;
-}
 
 template <typename T> struct AdjArgs {
     RimeArgs<T> R;
@@ -136,178 +109,6 @@ k_adj_vis(AdjArgs<T> A) {
             if (c < nc_ext) A.g_G[((uint64_t(t) * NA + a) * NC + c) * 8 + x] = s * Gacc[i];
         }
     }
-}
-
-// ---------------------------------------------------------------------
-// Per-item backward of J = E K sqrt(B) onto (l, m, I, Q, U, V, alpha).
-
-template <typename T>
-__device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const double *Luvw,
-                                                 const double *Lpa, const double *Lch, int t,
-                                                 int s, int a, int c, int k, const J4<T> &Jb,
-                                                 double *g /* [7] */) {
-    const mbx_dims &d = A.d;
-    const uint64_t NT = d.ntime.local_size, NA = d.na.local_size, NC = d.nchan.local_size;
-    const double l0 = A.lm[2 * uint64_t(s)], m0 = A.lm[2 * uint64_t(s) + 1];
-    const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
-    const double nm1 = n - 1.0;
-    // ---- beam value and its derivatives along the grid fractions
-    const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
-    const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
-    const double ps = Lpa[2 * a], pc = Lpa[2 * a + 1];
-    const double al = double(asc[0]), am = double(asc[1]);
-    const double lr = (l0 * pc - m0 * ps + double(pet[0])) * al;
-    const double mr = (l0 * ps + m0 * pc + double(pet[1])) * am;
-    const double LW1 = double(d.beam_lw.local_size - 1), MH1 = double(d.beam_mh.local_size - 1);
-    const double vl_raw = A.lscale * (lr - A.ll), vm_raw = A.mscale * (mr - A.lmin);
-    const double vl = fmin(LW1, fmax(0.0, vl_raw)), vm = fmin(MH1, fmax(0.0, vm_raw));
-    const bool okl = vl_raw >= 0.0 && vl_raw <= LW1, okm = vm_raw >= 0.0 && vm_raw <= MH1;
-    const double gl0 = floor(vl), gm0 = floor(vm);
-    const T ld = T(vl - gl0), md = T(vm - gm0), chd = T(Lch[CH * k + 2]);
-    const uint32_t MH = d.beam_mh.local_size, NUD = d.beam_nud.local_size;
-    const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
-    const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const uint32_t ic0 = uint32_t(Lch[CH * k]), ic1 = uint32_t(Lch[CH * k + 1]);
-    cx<T> S[4], dSl[4], dSm[4];
-    T Am[4], dAl[4], dAm[4];
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        S[x] = dSl[x] = dSm[x] = {T(0), T(0)};
-        Am[x] = dAl[x] = dAm[x] = T(0);
-    }
-#pragma unroll 1
-    for (int tap = 0; tap < 8; ++tap) {
-        const bool hl = tap & 1, hm = (tap >> 1) & 1, hc = tap >> 2;
-        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0, ic = hc ? ic1 : ic0;
-        const T wl = hl ? ld : T(1) - ld, wm = hm ? md : T(1) - md, wc = hc ? chd : T(1) - chd;
-        const T w = wl * wm * wc;
-        const T wdl = (hl ? T(1) : T(-1)) * wm * wc;
-        const T wdm = wl * (hm ? T(1) : T(-1)) * wc;
-        const BeamTap<T> *e = A.beam + ((uint64_t(il * MH + im) * NUD) + ic) * 4;
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            const BeamTap<T> v = e[x];
-            S[x].re = fma(w, v.re, S[x].re);     S[x].im = fma(w, v.im, S[x].im);
-            dSl[x].re = fma(wdl, v.re, dSl[x].re); dSl[x].im = fma(wdl, v.im, dSl[x].im);
-            dSm[x].re = fma(wdm, v.re, dSm[x].re); dSm[x].im = fma(wdm, v.im, dSm[x].im);
-            Am[x] = fma(w, v.amp, Am[x]);
-            dAl[x] = fma(wdl, v.amp, dAl[x]);
-            dAm[x] = fma(wdm, v.amp, dAm[x]);
-        }
-    }
-    J4<T> E, dEl, dEm;
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        const T n2 = S[x].re * S[x].re + S[x].im * S[x].im;
-        if (n2 > T(0) && isfinite(n2)) {
-            const T inv = T(1) / sqrt(n2);
-            const T g = Am[x] * inv;
-            E.v[x] = {S[x].re * g, S[x].im * g};
-            // dE = dS g + S dA / |S| - S A Re(conj(S) dS) / |S|^3
-            const T rl = (S[x].re * dSl[x].re + S[x].im * dSl[x].im) * inv * inv;
-            const T rm = (S[x].re * dSm[x].re + S[x].im * dSm[x].im) * inv * inv;
-            const T cl = dAl[x] * inv - g * rl, cm = dAm[x] * inv - g * rm;
-            dEl.v[x] = {dSl[x].re * g + S[x].re * cl, dSl[x].im * g + S[x].im * cl};
-            dEm.v[x] = {dSm[x].re * g + S[x].re * cm, dSm[x].im * g + S[x].im * cm};
-        } else {
-            E.v[x] = {S[x].re * Am[x], S[x].im * Am[x]};
-            dEl.v[x] = {dSl[x].re * Am[x] + S[x].re * dAl[x], dSl[x].im * Am[x] + S[x].im * dAl[x]};
-            dEm.v[x] = {dSm[x].re * Am[x] + S[x].re * dAm[x], dSm[x].im * Am[x] + S[x].im * dAm[x]};
-        }
-    }
-    // ---- phase and sqrt(B)
-    const double *xa = Luvw + 3 * a;
-    const double f = Lch[CH * k + 3];
-    const double phi = -A.two_pi_over_c * f * (xa[2] * nm1 + xa[1] * m0 + xa[0] * l0);
-    T sn, cs;
-    PhaseCS<T>::eval(phi, &sn, &cs);
-    const cx<T> K = {cs, sn};
-    const T *St = A.stokes + (uint64_t(s) * NT + t) * 4;
-    const T I = St[0], Q = St[1], U = St[2], V = St[3];
-    const double lnr = Lch[CH * k + 4];
-    const T P = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * lnr);
-    const T rP = sqrt(P);
-    const T s0 = sqrt(I * I - Q * Q - U * U - V * V);
-    const T t0 = sqrt(T(2) * I + T(2) * s0);
-    J4<T> B0s;                            // sqrt(B0), B0 = unscaled brightness
-    B0s.v[0] = {I + Q + s0, T(0)}; B0s.v[1] = {U, V}; B0s.v[2] = {U, -V}; B0s.v[3] = {I - Q + s0, T(0)};
-    const T it0 = t0 != T(0) ? T(1) / t0 : T(0);
-    if (t0 != T(0)) B0s = jscale(B0s, it0);
-    const J4<T> sB = jscale(B0s, rP);      // sqrt(B) = sqrt(P) sqrt(B0)
-    J4<T> X;
-#pragma unroll
-    for (int x = 0; x < 4; ++x) X.v[x] = cmul(K, sB.v[x]);
-    // ---- adjoints
-    const J4<T> Eb = jmul_h(Jb, X);                 // Jb X^H
-    const J4<T> Xb = jmul(jherm(E), Jb);            // E^H Jb
-    const T g_ld = T(2) * jdot_re(Eb, dEl);
-    const T g_md = T(2) * jdot_re(Eb, dEm);
-    // Kb = sum Xb conj(sqrtB); d chi^2/d phi = -2 Im(K conj(Kb))
-    cx<T> Kb = {T(0), T(0)};
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        Kb.re += Xb.v[x].re * sB.v[x].re + Xb.v[x].im * sB.v[x].im;
-        Kb.im += Xb.v[x].im * sB.v[x].re - Xb.v[x].re * sB.v[x].im;
-    }
-    const T g_phi = T(-2) * (K.im * Kb.re - K.re * Kb.im);
-    J4<T> Zb;                                       // sqrtB adjoint = conj(K) Xb
-#pragma unroll
-    for (int x = 0; x < 4; ++x) Zb.v[x] = cmul(cx<T>{K.re, -K.im}, Xb.v[x]);
-    // alpha: d sqrtB / d alpha = lnr / 2 sqrtB
-    g[6] = double(lnr) * double(jdot_re(Zb, sB));
-    // stokes through sqrt(B0): d sqrtB0/dx = (dB0/dx + ds/dx 1)/t - sqrtB0 (dx_I + ds/dx)/t^2
-    if (t0 != T(0) && s0 != T(0)) {
-        const T ZI = Zb.v[0].re + Zb.v[3].re;
-        const T ZB = jdot_re(Zb, B0s);
-        const T rdB[4] = {Zb.v[0].re + Zb.v[3].re, Zb.v[0].re - Zb.v[3].re,
-                          Zb.v[1].re + Zb.v[2].re, Zb.v[1].im - Zb.v[2].im};
-        const T is0 = T(1) / s0;
-        const T dsx[4] = {I * is0, -Q * is0, -U * is0, -V * is0};
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            const T dt = (x == 0 ? T(1) : T(0)) + dsx[x];
-            g[2 + x] = 2.0 * double(rP) * double((rdB[x] + dsx[x] * ZI) * it0 - ZB * dt * it0 * it0);
-        }
-    } else {
-        g[2] = g[3] = g[4] = g[5] = 0.0;
-    }
-    // l, m: phase and beam-weight paths
-    const double dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * l0 / n);
-    const double dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * m0 / n);
-    const double dvl_dl = okl ? A.lscale * al * pc : 0.0, dvl_dm = okl ? -A.lscale * al * ps : 0.0;
-    const double dvm_dl = okm ? A.mscale * am * ps : 0.0, dvm_dm = okm ? A.mscale * am * pc : 0.0;
-    g[0] = double(g_phi) * dphi_dl + double(g_ld) * dvl_dl + double(g_md) * dvm_dl;
-    g[1] = double(g_phi) * dphi_dm + double(g_ld) * dvl_dm + double(g_md) * dvm_dm;
-}
-
-// Derivatives of the shape factor f of one pair with respect to the three
-// shape parameters of its source (gaussian el, em, R or sersic e1, e2, R).
-template <typename T>
-__device__ __forceinline__ T shape_factor_grad(int ty, T u, T v, T p0, T p1, T p2, T kf,
-                                               T *df /* [3] */) {
-    if (ty == 1) {
-        // x = (u em - v el) k R, y = (u el + v em) k, f = exp(-(x^2 + y^2))
-        const T a = u * p1 - v * p0, bq = u * p0 + v * p1;
-        const T x = a * kf * p2, y = bq * kf;
-        const T f = exp(-(x * x + y * y));
-        const T m2f = T(-2) * f;
-        df[0] = m2f * (x * (-v * kf * p2) + y * (u * kf));
-        df[1] = m2f * (x * (u * kf * p2) + y * (v * kf));
-        df[2] = m2f * (x * a * kf);
-        return f;
-    }
-    // sersic: p2 holds R' = R / (1 - e1^2 - e2^2); D = 1 + x^2 + y^2, f = D^-3/2
-    const T den = T(1) - p0 * p0 - p1 * p1;
-    const T x0 = u * (T(1) + p0) + v * p1, y0 = u * p1 + v * (T(1) - p0);
-    const T x = x0 * kf * p2, y = y0 * kf * p2;
-    const T D = T(1) + x * x + y * y;
-    const T f = T(1) / (D * sqrt(D));
-    const T c = T(-3) * f / D;                        // df/dx = c x, df/dy = c y
-    const T dR1 = p2 * T(2) * p0 / den, dR2 = p2 * T(2) * p1 / den;   // dR'/de1, dR'/de2
-    df[0] = c * (x * (u * kf * p2 + x0 * kf * dR1) + y * (-v * kf * p2 + y0 * kf * dR1));
-    df[1] = c * (x * (v * kf * p2 + x0 * kf * dR2) + y * (u * kf * p2 + y0 * kf * dR2));
-    df[2] = c * (x * x0 * kf + y * y0 * kf) / den;
-    return f;
 }
 
 // ---------------------------------------------------------------------

@@ -1,0 +1,163 @@
+"""Batched evaluation (one persistent launch over many samples) and
+HIP-graph replay.
+
+Each sample's chi^2 and gradients must equal the numpy / torch float64
+oracle evaluated on that sample alone (float64: 1e-10 relative), and the
+launch-shape variants the autotuner picks from must all agree.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import rime_np, rime_torch, synth
+
+pytestmark = pytest.mark.gpu
+
+COUNTS = [(3, 1, 0), (0, 2, 0), (7, 2, 1), (1, 0, 0), (2, 3, 2)]
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def make_batch(na=7, ntime=3, nchan=4, counts=COUNTS, seed=8, dtype=np.float64, weights=False):
+    shared = synth.make_inputs(na, ntime, nchan, 0, 0, 0, beam=(9, 8, 6), seed=seed,
+                               flag_fraction=0.3)
+    r = np.random.default_rng(seed)
+    if weights:
+        shared['weight_vector'] = r.uniform(0.5, 2.0, shared['weight_vector'].shape)
+    shared['sigma_sqrd'] = 1.3
+    samples = []
+    for i, c in enumerate(counts):
+        s = synth.make_inputs(na, ntime, nchan, *c, beam=(2, 2, 2), seed=seed + 100 + i)
+        s['sersic_shape'][:2] = r.uniform(-0.3, 0.3, (2, c[2]))
+        samples.append({k: s[k] for k in ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape')})
+        samples[-1]['counts'] = c
+    # observed: the first sample's model plus noise, so chi^2 values differ
+    full = dict(shared)
+    full.update({k: samples[0][k] for k in ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape')})
+    full['flag'] = np.zeros_like(shared['flag'])
+    full['observed_vis'] = np.zeros_like(shared['observed_vis'])
+    m = rime_np.rime(full, *counts[0])['model_vis']
+    shared['observed_vis'] = m + 0.1 * (r.standard_normal(m.shape) + 1j * r.standard_normal(m.shape))
+    return shared, samples
+
+
+def sample_inputs(shared, s):
+    a = dict(shared)
+    a.update({k: s[k] for k in ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape')})
+    return a
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('weights', [False, True])
+def test_batched_matches_oracle_per_sample(dtype, weights):
+    from montblanc_amd import batched
+    shared, samples = make_batch(dtype=dtype, weights=weights)
+    print('batched', np.dtype(dtype).name, weights, flush=True)
+    b = batched.Batch(shared, samples, dtype=dtype)
+    X2, g = batched.evaluate(b, gradients=True, use_weight_vector=weights, tune=None)
+    torch.cuda.synchronize()
+    X2 = X2.cpu().numpy()
+    g = {k: v.cpu().numpy() for k, v in g.items()}
+    tol = 1e-10 if dtype == np.float64 else 1e-4
+    for k, s in enumerate(samples):
+        a = synth.upcast(synth.as_dtype(sample_inputs(shared, s), dtype))
+        ref = rime_torch.gradients(a, *s['counts'], use_weight_vector=weights, sigma_sqrd=1.3)
+        assert abs(X2[k] - ref['X2']) <= max(tol, 1e-6 if dtype == np.float32 else 0) * ref['X2']
+        o0, o1 = b.src_offset[k], b.src_offset[k + 1]
+        for name in ('lm', 'stokes', 'alpha'):
+            assert rel(g[name][o0:o1], ref[name]) < tol, (k, name)
+        if s['counts'][1]:
+            gs = b.split_shapes(torch.from_numpy(g['gauss_shape']), 'gauss')[k].numpy()
+            assert rel(gs, ref['gauss_shape']) < tol, (k, 'gauss')
+        if s['counts'][2]:
+            ss = b.split_shapes(torch.from_numpy(g['sersic_shape']), 'sersic')[k].numpy()
+            assert rel(ss, ref['sersic_shape']) < tol, (k, 'sersic')
+
+
+@pytest.mark.parametrize('tune', [None, {'workgroups': 1}, {'workgroups': 3, 'channels_per_item': 2},
+                                  {'sources_per_tile': 2, 'workgroups': 17}])
+def test_batched_launch_shapes_agree(tune):
+    """Every launch shape (down to one persistent workgroup working through
+    the whole queue) gives the same results."""
+    from montblanc_amd import batched
+    shared, samples = make_batch(na=9, ntime=2, nchan=5, seed=21)
+    b = batched.Batch(shared, samples)
+    X2, g = batched.evaluate(b, gradients=True, tune=None)
+    X2 = X2.clone()
+    g = {k: v.clone() for k, v in g.items()}
+    X2b, gb = batched.evaluate(b, gradients=True, tune=tune)
+    assert rel(X2b.cpu(), X2.cpu()) < 1e-12
+    for k in g:
+        assert rel(gb[k].cpu(), g[k].cpu()) < 1e-11, k
+
+
+def test_batched_64_antennas_matches_single_model_path():
+    """At 64 antennas (2016 baselines) the batched float32 results equal the
+    single-model forward and adjoint run per sample."""
+    from montblanc_amd import batched, rime
+    counts = [(40, 10, 0), (5, 2, 0), (200, 60, 0)]
+    shared, samples = make_batch(na=64, ntime=2, nchan=3, counts=counts, seed=33)
+    b = batched.Batch(shared, samples, dtype=np.float32)
+    X2, g = batched.evaluate(b, gradients=True, tune=None)
+    torch.cuda.synchronize()
+    for k, s in enumerate(samples):
+        arrays = rime.to_device(synth.as_dtype(sample_inputs(shared, s), np.float32))
+        out = rime.forward(arrays, s['counts'], keep_ekb=True)
+        gg = rime.adjoint(arrays, s['counts'], out['ekb_vis'])
+        torch.cuda.synchronize()
+        assert abs(X2[k].item() - out['X2'].item()) < 1e-6 * out['X2'].item()
+        o0, o1 = b.src_offset[k], b.src_offset[k + 1]
+        for name in ('lm', 'stokes', 'alpha'):
+            assert rel(g[name][o0:o1].cpu(), gg[name].cpu()) < 1e-4, (k, name)
+
+
+def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
+    from montblanc_amd import autotune, batched
+    monkeypatch.setenv('MBX_AUTOTUNE_CACHE', str(tmp_path / 'tune.json'))
+    autotune._MEM.clear()
+    shared, samples = make_batch(na=9, ntime=4, nchan=3, seed=5)
+    b = batched.Batch(shared, samples)
+    choice = autotune.batched_tuning(b, gradients=False)
+    assert set(choice) == {'workgroups', 'channels_per_item', 'sources_per_tile'}
+    assert (tmp_path / 'tune.json').exists()
+    autotune._MEM.clear()
+    assert autotune.batched_tuning(b, gradients=False) == choice   # from disk, no timing
+    X2 = batched.evaluate(b, tune='auto').clone()
+    X2n = batched.evaluate(b, tune=None)
+    assert rel(X2.cpu(), X2n.cpu()) < 1e-12
+
+
+def test_graph_replay_matches_eager():
+    """Forward and batched launches captured in a HIP graph: every replay,
+    with eager work and input changes between replays, equals the eager
+    result for the current inputs."""
+    from montblanc_amd import batched, graph, rime
+    counts = (3, 2, 1)
+    a = synth.make_inputs(8, 3, 4, *counts, beam=(9, 8, 6), seed=12, flag_fraction=0.2)
+    arrays = rime.to_device(a)
+    out = rime.alloc_outputs(arrays)
+    g = graph.Graphed(lambda: rime.forward(arrays, counts, outputs=out))
+    for rep in range(3):
+        arrays['stokes'].mul_(1.0 + 0.1 * rep)                 # eager change of an input
+        eager = rime.forward(arrays, counts)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert rel(out['model_vis'].cpu(), eager['model_vis'].cpu()) < 1e-13
+        assert abs(out['X2'].item() - eager['X2'].item()) < 1e-12 * eager['X2'].item()
+    shared, samples = make_batch(seed=40)
+    b = batched.Batch(shared, samples)
+    gb = graph.Graphed(lambda: batched.evaluate(b, gradients=True, tune=None))
+    for rep in range(3):
+        b.arrays['stokes'].mul_(1.0 + 0.05 * rep)
+        X2e, ge = batched.evaluate(b, gradients=True, tune=None)
+        X2e, lme = X2e.clone(), ge['lm'].clone()
+        torch.cuda.synchronize()
+        gb.replay()
+        torch.cuda.synchronize()
+        assert rel(b.X2.cpu(), X2e.cpu()) < 1e-12
+        assert rel(b.grads_buffers()['lm'].cpu(), lme.cpu()) < 1e-11
