@@ -1,0 +1,333 @@
+"""Memory planner and chunked solver for problems larger than one device.
+
+The planner keeps the reference's reduction-ladder semantics
+(montblanc/util/__init__.py:166-285 viable_dim_config, driven by
+montblanc/impl/rime/v5/CompositeRimeSolver.py:644-718 _thread_budget):
+reductions are applied in order until the device arrays fit the budget;
+'ntime=20' lowers a dimension to 20 (never raises it), 'nchan=50%' halves
+it repeatedly, 'nbl=1&na=2' lowers several at once, and MemoryError is
+raised when the ladder runs out.
+
+What changes for one MI355X:
+* the budget is the device's free memory (hipMemGetInfo through torch)
+  less a reserve, or cfg['mem_budget'];
+* the arrays budgeted are those the fused path actually keeps on the
+  device -- the v4 inputs and outputs, the pre-gain sums when gradients
+  are wanted and the prepared beam cube -- not the reference's
+  (nsrc, ntime, na, nchan, 4) jones scratch, which the fused kernel never
+  materialises (the staged pipeline still budgets it);
+* one solver per device (no per-device sub-solver streams), so the ladder
+  first halves the time chunk, then batches sources (staged pipeline
+  only: the fused path's source arrays are small), then halves channels.
+"""
+
+import logging
+import math
+import re
+from collections import OrderedDict
+
+import numpy as np
+
+from . import schema, src_types
+from .config import RimeSolverConfig as Options
+
+log = logging.getLogger('montblanc')
+
+ONE_MB = 1024 ** 2
+DEVICE_RESERVE = 512 * ONE_MB
+
+_RED_RE = re.compile(r'^\s*(?P<name>[A-Za-z_0-9]+)\s*(=\s*(?P<value>\d+)\s*(?P<percent>%)?)?\s*$')
+
+FUSED_ARRAYS = ('uvw', 'antenna1', 'antenna2', 'frequency', 'ref_frequency', 'parallactic_angles',
+                'point_errors', 'antenna_scaling', 'E_beam', 'G_term', 'lm', 'stokes', 'alpha',
+                'gauss_shape', 'sersic_shape', 'flag', 'weight_vector', 'observed_vis', 'model_vis',
+                'chi_sqrd_result')
+
+
+def _itemsize(dtype, ft):
+    if isinstance(dtype, str):
+        return {'ft': np.dtype(ft).itemsize, 'ct': 2 * np.dtype(ft).itemsize}[dtype]
+    return np.dtype(dtype).itemsize
+
+
+def array_bytes(shape, dtype, dims, ft):
+    n = 1
+    for s in shape:
+        n *= dims[s] if isinstance(s, str) else int(s)
+    return n * _itemsize(dtype, ft)
+
+
+def device_bytes(dims, ft, pipeline='fused', gradients=False):
+    """Device bytes the chosen pipeline needs for problem dims."""
+    total = 0
+    for name, shape, dtype, _d, _t, scratch in schema.ARRAYS:
+        if pipeline == 'fused' and (scratch or name not in FUSED_ARRAYS):
+            continue
+        total += array_bytes(shape, dtype, dims, ft)
+    vis = dims['ntime'] * dims['nbl'] * dims['nchan'] * 4 * 2 * np.dtype(ft).itemsize
+    beam = dims['beam_lw'] * dims['beam_mh'] * dims['beam_nud'] * 4 * 4 * np.dtype(ft).itemsize
+    total += beam
+    if gradients:
+        total += 2 * vis          # pre-gain sums + the adjoint's Vb workspace
+    return total
+
+
+def apply_reductions(budget, dims, ladder, bytes_of):
+    """viable_dim_config semantics: returns (viable, reduced dims dict)."""
+    ladder = list(ladder)
+    T = dict(dims)
+    modified = OrderedDict()
+    if budget < 0:
+        budget = 0
+    used = bytes_of(T)
+    while used > budget:
+        if not ladder:
+            return False, modified, T
+        step = ladder.pop(0)
+        for part in step.strip().split('&'):
+            m = _RED_RE.match(part)
+            if not m:
+                raise ValueError("%s is an invalid dimension reduction string; valid strings are "
+                                 "e.g. 'ntime', 'ntime=20' or 'ntime=20%%'" % part)
+            name, value, pct = m.group('name'), m.group('value'), m.group('percent')
+            value = 1 if value is None else int(value)
+            if pct == '%':
+                value = int(T[name] * value / 100.0)
+                if value < 1:
+                    value = 1
+                else:
+                    ladder.insert(0, part)
+            if T[name] > value:
+                T[name] = value
+                modified[name] = value
+            else:
+                log.info('Ignored reduction of %s of size %s to %s.', name, T[name], value)
+        used = bytes_of(T)
+    return True, modified, T
+
+
+def default_ladder(source_batch_size, pipeline='fused'):
+    ladder = ['ntime=50%']
+    if pipeline != 'fused':
+        srcs = '&'.join('%s=%d' % (n, source_batch_size)
+                        for n in ['nsrc'] + src_types.source_nr_vars())
+        ladder.append(srcs)
+    ladder.append('nchan=50%')
+    return ladder
+
+
+def device_budget(cfg=None, device=None):
+    if cfg is not None and cfg.get('mem_budget') is not None:
+        return int(cfg['mem_budget'])
+    import torch
+    free, _total = torch.cuda.mem_get_info(device)
+    return int(free) - DEVICE_RESERVE
+
+
+def plan(dims, ft, budget, pipeline='fused', gradients=False, source_batch_size=500, ladder=None):
+    """Reduced per-chunk dims that fit ``budget`` bytes.
+
+    Raises MemoryError, listing the largest arrays, when no reduction fits
+    (CompositeRimeSolver.py:703-716).
+    """
+    ladder = default_ladder(source_batch_size, pipeline) if ladder is None else ladder
+
+    def bytes_of(T):
+        T2 = dict(T)
+        T2['nvis'] = T2['ntime'] * T2['nbl'] * T2['nchan']
+        T2['npolchan'] = T2['nchan'] * T2.get('npol', 4)
+        return device_bytes(T2, ft, pipeline, gradients)
+    viable, modified, T = apply_reductions(budget, dims, ladder, bytes_of)
+    if not viable:
+        worst = sorted(((array_bytes(s, dt, dict(T, npolchan=4 * T['nchan']), ft), n)
+                        for n, s, dt, *_ in schema.ARRAYS), reverse=True)[:5]
+        raise MemoryError(
+            "Tried reducing the problem size by setting '%s' on all arrays, but the required "
+            "memory of %d bytes does not fit the budget of %d bytes. Largest arrays: %s"
+            % (', '.join('%s=%s' % kv for kv in modified.items()), bytes_of(T), budget,
+               ', '.join('%s %d' % (n, b) for b, n in worst)))
+    return modified, T, bytes_of(T)
+
+
+def chunks(n, size):
+    """[start, end) ranges of at most ``size`` covering [0, n)."""
+    size = max(1, int(size))
+    return [(s, min(n, s + size)) for s in range(0, n, size)]
+
+
+class ChunkedRimeSolver(object):
+    """The v4 solver interface over problems that do not fit one device.
+
+    Full-size arrays live on the host (transfer_<name>/retrieve_<name>
+    work on them); solve() runs the fused forward chunk by chunk over time
+    (and channels, and source batches through the nsrc extents when the
+    plan asks for them), writing model visibilities back to the host array
+    (overwritten, or accumulated with vis_write_mode='sum') and summing
+    X2 over chunks -- the host side of the reference's v5 solve loop
+    (CompositeRimeSolver.py:1027-1215), synchronous and on one device.
+    """
+
+    def __init__(self, slvr_cfg):
+        import torch
+        from . import rime
+        self._cfg = slvr_cfg
+        self.ft = np.float32 if slvr_cfg[Options.DTYPE] == Options.DTYPE_FLOAT else np.float64
+        self.ct = np.complex64 if self.ft == np.float32 else np.complex128
+        self._use_w = bool(slvr_cfg.get(Options.WEIGHT_VECTOR))
+        self._residual = slvr_cfg.get(Options.VISIBILITY_OUTPUT) == Options.VISIBILITY_OUTPUT_RESIDUALS
+        self._sum = slvr_cfg.get(Options.VISIBILITY_WRITE_MODE) == Options.VISIBILITY_WRITE_MODE_SUM
+        from .solver import RimeSolver, resolve_device
+        self.device = resolve_device(slvr_cfg.get(Options.CONTEXT))
+        # a host-side twin holds dims, properties and the full arrays
+        cfg = dict(slvr_cfg)
+        self._host = _HostArrays(cfg, self.ft, self.ct)
+        self.X2 = self.ft(0.0)
+        self.last_plan = None
+        self._rime = rime
+        self._torch = torch
+
+    def __getattr__(self, name):
+        h = self.__dict__.get('_host')
+        if h is None:
+            raise AttributeError(name)
+        if name.startswith('transfer_'):
+            return lambda ary, _n=name[9:]: h.transfer(_n, ary)
+        if name.startswith('retrieve_'):
+            return lambda _n=name[9:]: h.arrays[_n].copy()
+        if name.startswith('set_'):
+            return lambda v, _n=name[4:]: h.props.__setitem__(_n, self.ft(v))
+        if name in h.props:
+            return h.props[name]
+        if name in h.arrays:
+            return h.arrays[name]
+        if name.startswith('dim_'):
+            return getattr(h, name)
+        raise AttributeError(name)
+
+    def plan(self):
+        h = self._host
+        dims = dict(h.dims)
+        budget = device_budget(self._cfg, self.device)
+        modified, T, need = plan(dims, self.ft, budget,
+                                 source_batch_size=self._cfg.get(Options.SOURCE_BATCH_SIZE, 500))
+        self.last_plan = dict(modified=dict(modified), chunk_dims=T, bytes=need, budget=budget)
+        log.info('montblanc planner: %s per chunk (%d bytes of %d)', dict(modified), need, budget)
+        return T
+
+    def solve(self):
+        rime, torch = self._rime, self._torch
+        h = self._host
+        T = self.plan()
+        nt, nc = h.dims['ntime'], h.dims['nchan']
+        counts = (h.dims['npsrc'], h.dims['ngsrc'], h.dims['nssrc'])
+        x2 = 0.0
+        model = h.arrays['model_vis']
+        chi = h.arrays['chi_sqrd_result']
+        if not self._sum:
+            model[...] = 0
+        for t0, t1 in chunks(nt, T['ntime']):
+            for c0, c1 in chunks(nc, T['nchan']):
+                a = h.slice(t0, t1, c0, c1)
+                arrays = rime.to_device(a, self.device)
+                out = rime.forward(arrays, counts, residual=self._residual,
+                                   use_weight_vector=self._use_w,
+                                   sigma_sqrd=float(h.props['sigma_sqrd']))
+                torch.cuda.synchronize(self.device)
+                mv = out['model_vis'].cpu().numpy()
+                if self._sum:
+                    model[t0:t1, :, c0:c1] += mv
+                else:
+                    model[t0:t1, :, c0:c1] = mv
+                chi[t0:t1, :, c0:c1] = out['chi_sqrd_result'].cpu().numpy()
+                x2 += float(out['X2'].item())
+                del arrays, out
+        self.X2 = self.ft(x2)
+        h.props['X2'] = self.X2
+        return False
+
+    def initialise(self):
+        return True
+
+    def shutdown(self):
+        return True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+class _HostArrays(object):
+    """Full-size numpy arrays, dims and properties of the v4 schema."""
+
+    def __init__(self, cfg, ft, ct):
+        from .solver import nr_of_baselines
+        autocor = cfg.get(Options.AUTO_CORRELATIONS, False)
+        nr = src_types.sources_to_nr_vars(cfg[Options.SOURCES])
+        na = cfg[Options.NA]
+        self.dims = dict(ntime=cfg[Options.NTIME], na=na, nchan=cfg[Options.NCHAN], npol=4,
+                         nbl=cfg.get(Options.NBL) or nr_of_baselines(na, autocor),
+                         beam_lw=cfg[Options.E_BEAM_WIDTH], beam_mh=cfg[Options.E_BEAM_HEIGHT],
+                         beam_nud=cfg[Options.E_BEAM_DEPTH], **nr)
+        self.dims['nsrc'] = sum(nr.values())
+        self.dims['npolchan'] = 4 * self.dims['nchan']
+        self.dims['nvis'] = self.dims['ntime'] * self.dims['nbl'] * self.dims['nchan']
+        self.ft, self.ct = ft, ct
+        self.props = {n: ft(v) for n, _t, v in schema.PROPERTIES}
+        self.arrays = {}
+        a1, a2 = np.triu_indices(na, 0 if autocor else 1)
+        nbl = self.dims['nbl']
+        for name, shape, dtype, default, _test, scratch in schema.ARRAYS:
+            if scratch:
+                continue
+            dt = {'ft': ft, 'ct': ct}.get(dtype, dtype) if isinstance(dtype, str) else dtype
+            shp = tuple(self.dims[s] if isinstance(s, str) else s for s in shape)
+            arr = np.zeros(shp, dtype=dt)
+            if name == 'antenna1':
+                arr[...] = a1[:nbl]
+            elif name == 'antenna2':
+                arr[...] = a2[:nbl]
+            elif name == 'frequency':
+                arr[...] = np.linspace(1e9, 2e9, shp[0])
+            elif isinstance(default, np.ndarray) or np.isscalar(default):
+                arr[...] = default
+            self.arrays[name] = arr
+
+    def transfer(self, name, ary):
+        ary = np.asarray(ary)
+        if ary.shape != self.arrays[name].shape:
+            raise ValueError('transfer_%s: shape %s does not match %s'
+                             % (name, ary.shape, self.arrays[name].shape))
+        if ary.dtype != self.arrays[name].dtype:
+            raise ValueError('transfer_%s: dtype %s does not match %s'
+                             % (name, ary.dtype, self.arrays[name].dtype))
+        self.arrays[name][...] = ary
+
+    def dim_local_size(self, *names):
+        v = tuple(self.dims[n] for n in names)
+        return v[0] if len(v) == 1 else v
+
+    dim_global_size = dim_local_size
+
+    def slice(self, t0, t1, c0, c1):
+        """The chunk's arrays (time slice t0:t1, channel slice c0:c1)."""
+        A = self.arrays
+        out = {}
+        for name, arr in A.items():
+            if name in ('model_vis', 'chi_sqrd_result'):
+                continue
+            shape = dict((n, s) for n, s, *_ in ((x[0], x[1]) for x in schema.ARRAYS))[name]
+            idx = []
+            for s in shape:
+                if s == 'ntime':
+                    idx.append(slice(t0, t1))
+                elif s == 'nchan':
+                    idx.append(slice(c0, c1))
+                else:
+                    idx.append(slice(None))
+            out[name] = np.ascontiguousarray(arr[tuple(idx)])
+        for k in ('beam_ll', 'beam_lm', 'beam_lfreq', 'beam_ul', 'beam_um', 'beam_ufreq',
+                  'gauss_scale', 'two_pi_over_c', 'sigma_sqrd'):
+            out[k] = float(self.props[k])
+        return out
