@@ -39,6 +39,8 @@ def parse():
     p.add_argument('--dtypes', default='f32,f64')
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-gradient', action='store_true')
+    p.add_argument('--large', action='store_true',
+                   help='also run the large single-model workload (about a minute)')
     return p.parse_args()
 
 
@@ -147,21 +149,90 @@ def time_gradient(w, dtype, steps, warmup):
             'gradient_over_forward': res['forward'] / res['forward_gradient']}
 
 
+def time_large(steps=1):
+    """Large single model: 64 antennas, 500 times, 256 channels, 10000
+    point + 2000 gaussian sources, beam 129x129x64, float32 with float64
+    phase. Forward and forward + full gradient, resident in HBM (the
+    planner's budget for one device needs no chunking); the float32 model
+    visibilities are compared with the float64 numpy oracle on a slice
+    (time 0, two channels, every baseline and source)."""
+    import numpy as np
+    import torch
+    from montblanc_amd import planner, rime
+    from oracle import rime_np, synth
+
+    w = dict(synth.WORKLOADS['large_64ant'])
+    counts = (w['npsrc'], w['ngsrc'], w['nssrc'])
+    a = synth.make_inputs(**w, seed=1004, vis_arrays=False)
+    arrays = rime.to_device(synth.as_dtype(a, np.float32))
+    nt, nbl, nc = w['ntime'], w['na'] * (w['na'] - 1) // 2, w['nchan']
+    g = torch.Generator(device='cuda').manual_seed(4)
+    arrays['observed_vis'] = torch.view_as_complex(
+        0.1 * torch.randn((nt, nbl, nc, 4, 2), generator=g, device='cuda')).contiguous()
+    arrays['weight_vector'] = torch.ones((nt, nbl, nc, 4), device='cuda')
+    arrays['flag'] = torch.zeros((nt, nbl, nc, 4), dtype=torch.uint8, device='cuda')
+    budget = torch.cuda.mem_get_info()[0]
+    dims = dict(ntime=nt, na=w['na'], nbl=nbl, nchan=nc, npol=4, npsrc=counts[0],
+                ngsrc=counts[1], nssrc=0, nsrc=sum(counts), beam_lw=w['beam'][0],
+                beam_mh=w['beam'][1], beam_nud=w['beam'][2])
+    mod, _, need = planner.plan(dims, np.float32, budget - planner.DEVICE_RESERVE, gradients=True)
+    out = rime.alloc_outputs(arrays, keep_ekb=True)
+    grads = rime.alloc_grads(arrays, counts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rime.forward(arrays, counts, outputs=out)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rime.forward(arrays, counts, outputs=out)
+        rime.adjoint(arrays, counts, out['ekb_vis'], grads=grads)
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / steps
+    # error against the float64 oracle on a slice
+    up = synth.upcast(synth.as_dtype(a, np.float32))
+    sl = dict(up)
+    sl['uvw'] = up['uvw'][:1]
+    sl['parallactic_angles'] = up['parallactic_angles'][:1]
+    sl['point_errors'] = up['point_errors'][:1, :, :2]
+    sl['antenna_scaling'] = up['antenna_scaling'][:, :2]
+    sl['frequency'], sl['ref_frequency'] = up['frequency'][:2], up['ref_frequency'][:2]
+    sl['stokes'], sl['alpha'] = up['stokes'][:, :1], up['alpha'][:, :1]
+    sl['antenna1'], sl['antenna2'] = up['antenna1'][:1], up['antenna2'][:1]
+    sl['G_term'] = up['G_term'][:1, :, :2]
+    V = rime_np.coherencies(sl, *counts, src_chunk=256)
+    M = rime_np.gekb_vis(V, sl['G_term'], sl['antenna1'], sl['antenna2'],
+                         np.zeros(V.shape, np.uint8), np.zeros(V.shape, np.complex128), False)
+    got = out['model_vis'][:1, :, :2].cpu().numpy()
+    err = float(np.abs(got - M).max() / np.abs(M).max())
+    n_terms = sum(counts) * nt * nbl * nc
+    del arrays, out, grads
+    torch.cuda.empty_cache()
+    return {'workload': 'large_64ant', 'dtype': 'f32', 'terms': n_terms,
+            'planner_reductions': dict(mod), 'device_bytes_planned': need,
+            'forward_s': t_fwd, 'forward_gradient_s': t_grad,
+            'forward_terms_per_s': n_terms / t_fwd, 'gradient_terms_per_s': n_terms / t_grad,
+            'gradient_over_forward': t_fwd / t_grad,
+            'max_rel_error_vs_f64_oracle': err,
+            'error_slice': 'time 0, channels 0-1, all 2016 baselines and 12000 sources'}
+
+
 def cpu_baseline(w):
-    """numpy restatement of the reference CPU chain on one time step and the
-    first 16 channels, all sources and baselines (~10-30 s single core)."""
+    """numpy restatement of the reference CPU chain on four time steps and
+    16 channels, all sources and baselines (~10-30 s on one core)."""
     import numpy as np
     from oracle import rime_np, synth
     sample = {k: v for k, v in w.items() if k != 'name'}
-    sample.update(ntime=1, nchan=16)
+    sample.update(ntime=4, nchan=16)
     a = synth.make_inputs(**sample, seed=1001)
     t0 = time.perf_counter()
     rime_np.rime(a, w['npsrc'], w['ngsrc'], w['nssrc'], src_chunk=64)
     dt = time.perf_counter() - t0
     nbl = w['na'] * (w['na'] - 1) // 2
-    n = (w['npsrc'] + w['ngsrc'] + w['nssrc']) * 1 * nbl * 16
+    n = (w['npsrc'] + w['ngsrc'] + w['nssrc']) * 4 * nbl * 16
     return dict(value=n / dt, unit='source-vis terms/s', cores=1, kind='port',
-                sample='%s with ntime=1, nchan=16 (all %d sources, %d baselines): %.3g terms '
+                sample='%s with ntime=4, nchan=16 (all %d sources, %d baselines): %.3g terms '
                        'in %.1f s, numpy float64, one process; host %s, os.cpu_count()=%d'
                        % (w.get('name', 'workload'), w['npsrc'] + w['ngsrc'] + w['nssrc'], nbl,
                           n, dt, cpu_model(), os.cpu_count() or 0))
@@ -222,6 +293,9 @@ def main():
               % (grad['forward_ms'], grad['forward_gradient_ms'], grad['gradient_over_forward']),
               file=sys.stderr, flush=True)
         line['gradient'] = grad
+    if args.large:
+        line['large'] = time_large()
+        print('# large: %s' % line['large'], file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(w)
     print(json.dumps(line), flush=True)

@@ -32,14 +32,14 @@ def ptr(t):
 
 
 def consts(solver):
-    return _lib.mbx_consts(float(solver.gauss_scale), float(solver.two_pi_over_c),
-                           float(solver.sigma_sqrd))
+    p = solver.exact_property
+    return _lib.mbx_consts(p('gauss_scale'), p('two_pi_over_c'), p('sigma_sqrd'))
 
 
 def beam_extents(solver):
-    return _lib.mbx_beam_extents(float(solver.beam_ll), float(solver.beam_lm),
-                                 float(solver.beam_lfreq), float(solver.beam_ul),
-                                 float(solver.beam_um), float(solver.beam_ufreq))
+    p = solver.exact_property
+    return _lib.mbx_beam_extents(p('beam_ll'), p('beam_lm'), p('beam_lfreq'),
+                                 p('beam_ul'), p('beam_um'), p('beam_ufreq'))
 
 
 def options(solver):

@@ -195,9 +195,10 @@ class ChunkedRimeSolver(object):
         if name.startswith('retrieve_'):
             return lambda _n=name[9:]: h.arrays[_n].copy()
         if name.startswith('set_'):
-            return lambda v, _n=name[4:]: h.props.__setitem__(_n, self.ft(v))
+            return lambda v, _n=name[4:]: h.props.__setitem__(_n, np.float64(v))
         if name in h.props:
-            return h.props[name]
+            # read back in the solver's type; kept (and used) in float64
+            return self.ft(h.props[name])
         if name in h.arrays:
             return h.arrays[name]
         if name.startswith('dim_'):
@@ -242,7 +243,7 @@ class ChunkedRimeSolver(object):
                 x2 += float(out['X2'].item())
                 del arrays, out
         self.X2 = self.ft(x2)
-        h.props['X2'] = self.X2
+        h.props['X2'] = np.float64(x2)
         return False
 
     def initialise(self):
@@ -274,7 +275,7 @@ class _HostArrays(object):
         self.dims['npolchan'] = 4 * self.dims['nchan']
         self.dims['nvis'] = self.dims['ntime'] * self.dims['nbl'] * self.dims['nchan']
         self.ft, self.ct = ft, ct
-        self.props = {n: ft(v) for n, _t, v in schema.PROPERTIES}
+        self.props = {n: np.float64(v) for n, _t, v in schema.PROPERTIES}
         self.arrays = {}
         a1, a2 = np.triu_indices(na, 0 if autocor else 1)
         nbl = self.dims['nbl']

@@ -241,15 +241,25 @@ class RimeSolver(object):
 
     # ---- properties ------------------------------------------------------
     def register_property(self, name, dtype, default):
+        """Property read back in the solver's type (slvr.two_pi_over_c is a
+        float32 in a float solver, as in the reference); the value as given
+        is kept in float64 and that is what the kernels receive, so the
+        float64 phase argument is not built on a float32-rounded 2 pi / C."""
         ft = self.type_dict().get(dtype, dtype) if isinstance(dtype, str) else dtype
         self._props[name] = ft
-        setattr(self, '_prop_' + name, ft(default))
+        setattr(self, '_prop_' + name, np.float64(default))
 
-        def setter(value, _n=name, _t=ft):
-            setattr(self, '_prop_' + _n, _t(value))
+        def setter(value, _n=name):
+            setattr(self, '_prop_' + _n, np.float64(value))
             if _n == 'X2':
                 self._x2_dev = None
         setattr(self, 'set_' + name, setter)
+
+    def exact_property(self, name):
+        """The float64 value of a property as it was given."""
+        if name == 'X2':
+            self._read_x2()
+        return float(getattr(self, '_prop_' + name))
 
     def __getattr__(self, name):
         # Array and property access as attributes (slvr.uvw, slvr.sigma_sqrd)
@@ -257,17 +267,16 @@ class RimeSolver(object):
         if name == 'X2' and '_props' in d:
             return self._read_x2()
         if '_props' in d and name in d['_props']:
-            return d['_prop_' + name]
+            return d['_props'][name](d['_prop_' + name])
         if '_arrays' in d and name in d['_arrays']:
             return self.tensor(name)
         raise AttributeError(name)
 
     def _read_x2(self):
         if self._x2_dev is not None:
-            v = float(self._x2_dev.item())
-            self._prop_X2 = self.ft(v)
+            self._prop_X2 = np.float64(self._x2_dev.item())
             self._x2_dev = None
-        return self._prop_X2
+        return self.ft(self._prop_X2)
 
     def x2_device_scalar(self):
         """Device float64 scalar the reduction writes into; X2 reads it."""

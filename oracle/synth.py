@@ -33,11 +33,13 @@ WORKLOADS = {
 
 
 def make_inputs(na, ntime, nchan, npsrc, ngsrc, nssrc, beam=(33, 33, 16), seed=1000,
-                flag_fraction=0.0, auto_correlations=False, observed='noise'):
+                flag_fraction=0.0, auto_correlations=False, observed='noise', vis_arrays=True):
     """Return a dict of float64/complex128 numpy arrays in the v4 schema.
 
     observed: 'noise' -> CN(0, 0.1) only; 'model' -> oracle model + noise
-    (only sensible for small shapes).
+    (only sensible for small shapes). vis_arrays=False leaves out the
+    (ntime, nbl, nchan, ...) arrays (observed_vis, weight_vector, flag) for
+    callers that generate them on the device.
     """
     rng = np.random.default_rng(seed)
     nsrc = npsrc + ngsrc + nssrc
@@ -70,6 +72,14 @@ def make_inputs(na, ntime, nchan, npsrc, ngsrc, nssrc, beam=(33, 33, 16), seed=1
     G[..., 3] = 1.0
     G += 0.01 * (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape))
     a['G_term'] = G
+    a['beam_ll'], a['beam_lm'], a['beam_lfreq'] = -0.02, -0.02, 1.39e9
+    a['beam_ul'], a['beam_um'], a['beam_ufreq'] = 0.02, 0.02, 1.43e9
+    a['gauss_scale'] = rime_np.GAUSS_SCALE
+    a['two_pi_over_c'] = rime_np.TWO_PI_OVER_C
+    a['sigma_sqrd'] = 1.0
+    a['_counts'] = (npsrc, ngsrc, nssrc)
+    if not vis_arrays:
+        return a
     a['weight_vector'] = np.ones((ntime, nbl, nchan, 4))
     if flag_fraction > 0:
         a['flag'] = (rng.uniform(0, 1, (ntime, nbl, nchan, 4)) < flag_fraction).astype(np.uint8)
@@ -77,11 +87,6 @@ def make_inputs(na, ntime, nchan, npsrc, ngsrc, nssrc, beam=(33, 33, 16), seed=1
         a['flag'] = np.zeros((ntime, nbl, nchan, 4), np.uint8)
     noise = 0.1 * (rng.standard_normal((ntime, nbl, nchan, 4))
                    + 1j * rng.standard_normal((ntime, nbl, nchan, 4)))
-    a['beam_ll'], a['beam_lm'], a['beam_lfreq'] = -0.02, -0.02, 1.39e9
-    a['beam_ul'], a['beam_um'], a['beam_ufreq'] = 0.02, 0.02, 1.43e9
-    a['gauss_scale'] = rime_np.GAUSS_SCALE
-    a['two_pi_over_c'] = rime_np.TWO_PI_OVER_C
-    a['sigma_sqrd'] = 1.0
     if observed == 'model':
         clean = dict(a)
         clean['flag'] = np.zeros_like(a['flag'])
@@ -90,7 +95,6 @@ def make_inputs(na, ntime, nchan, npsrc, ngsrc, nssrc, beam=(33, 33, 16), seed=1
         a['observed_vis'] = model + noise
     else:
         a['observed_vis'] = noise
-    a['_counts'] = (npsrc, ngsrc, nssrc)
     return a
 
 

@@ -48,7 +48,8 @@ def test_chunked_solver_matches_resident_solver_and_oracle(dtype):
     v4 = configure('v4', dtype, counts)
     load(v4, a, ft)
     v4.solve()
-    v5 = configure('v5', dtype, counts, budget=400 * 1024)
+    # about the whole problem's size: forces at least two time chunks
+    v5 = configure('v5', dtype, counts, budget=(400 if dtype == 'double' else 200) * 1024)
     load(v5, a, ft)
     v5.solve()
     plan = v5.last_plan
@@ -58,8 +59,17 @@ def test_chunked_solver_matches_resident_solver_and_oracle(dtype):
     m4, m5 = v4.retrieve_model_vis(), v5.retrieve_model_vis()
     assert np.abs(m5 - m4).max() <= tol * np.abs(m4).max()
     assert np.abs(m5 - ref['model_vis']).max() <= tol * np.abs(ref['model_vis']).max()
-    assert abs(float(v5.X2) - ref['X2']) <= tol * ref['X2']
-    assert abs(float(v5.X2) - float(v4.X2)) <= tol * ref['X2']
+    # chi^2 holds the model error propagated through the residual
+    # R = O - M (Cauchy-Schwarz): |dX2| <= 2 ||R|| tol ||M||; for float64
+    # this is within the plain relative bound as well
+    up = synth.upcast(synth.as_dtype(a, ft))
+    keep = up['flag'] == 0
+    R = (up['observed_vis'] - ref['model_vis']) * keep
+    bound = 2 * np.linalg.norm(R) * tol * np.linalg.norm(ref['model_vis'] * keep)
+    if dtype == 'double':
+        bound = min(bound, tol * ref['X2'])
+    assert abs(float(v5.X2) - ref['X2']) <= bound
+    assert abs(float(v5.X2) - float(v4.X2)) <= bound
 
 
 def test_sum_write_mode_accumulates():
