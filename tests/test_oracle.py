@@ -319,13 +319,65 @@ def test_chunking_is_invisible():
     np.testing.assert_allclose(r1, r2, rtol=1e-13, atol=1e-13)
 
 
-def test_planted_mistakes_are_caught():
-    """Plant plausible mistakes in a copy of the oracle's arithmetic and
-    check the pinning checks above would see them."""
-    a = synth.make_inputs(4, 2, 3, 1, 0, 0, beam=(5, 5, 4), seed=3)
-    good = rime_np.coherencies(a, 1, 0, 0)
-    swapped = dict(a, antenna1=a['antenna2'], antenna2=a['antenna1'])
-    bad = rime_np.coherencies(swapped, 1, 0, 0)
-    assert not np.allclose(good, bad)   # (p, q) order matters
-    conj_K = np.conj(rime_np.k_jones(a['uvw'], a['lm'], a['frequency']))
-    assert not np.allclose(conj_K, rime_np.k_jones(a['uvw'], a['lm'], a['frequency']))
+def _fails(check):
+    try:
+        check()
+    except AssertionError:
+        return True
+    return False
+
+
+def test_planted_mistakes_are_caught(monkeypatch):
+    """Plant plausible mistakes in the oracle (on the CPU, in this process
+    only) and check that the pinning checks above fail on each."""
+    # 1. a dropped term: the w (n - 1) part of the phase
+    good_k = rime_np.k_jones
+
+    def k_no_w(uvw, lm, frequency):
+        flat = uvw.copy()
+        flat[..., 2] = 0.0
+        return good_k(flat, lm, frequency)
+    monkeypatch.setattr(rime_np, 'k_jones', k_no_w)
+    assert _fails(test_k_jones_brute_force_and_unit_modulus)
+    monkeypatch.setattr(rime_np, 'k_jones', good_k)
+
+    # 2. a wrong sign: sqrt(B) = (B - s 1) / t
+    good_sqrt = rime_np.b_sqrt_jones
+
+    def sqrt_sign(B):
+        r = good_sqrt(B)
+        det = B[..., 0] * B[..., 3] - B[..., 1] * B[..., 2]
+        s_ = np.sqrt(det)
+        t_ = np.sqrt(B[..., 0] + B[..., 3] + 2 * s_)
+        t_ = np.where(t_ == 0, 1.0, t_)
+        r = B / t_[..., None]
+        r[..., 0] -= s_ / t_
+        r[..., 3] -= s_ / t_
+        return r
+    monkeypatch.setattr(rime_np, 'b_sqrt_jones', sqrt_sign)
+    assert _fails(test_b_sqrt_matches_scipy_sqrtm_and_invariants)
+    monkeypatch.setattr(rime_np, 'b_sqrt_jones', good_sqrt)
+
+    # 3. a transposed operand: J_p J_q^T instead of J_p J_q^H
+    good_mul_h = rime_np.jones_mul_h
+
+    def mul_t(A, B):
+        return good_mul_h(A, np.conj(B))
+    monkeypatch.setattr(rime_np, 'jones_mul_h', mul_t)
+    assert _fails(lambda: test_rime_brute_force(False, False))
+    monkeypatch.setattr(rime_np, 'jones_mul_h', good_mul_h)
+
+    # 4. swapped antennas (V_qp instead of V_pq): the Fourier relation fails
+    good_cs = rime_np.coherency_sum
+
+    def swapped(jones, antenna1, antenna2, *args, **kw):
+        return good_cs(jones, antenna2, antenna1, *args, **kw)
+    monkeypatch.setattr(rime_np, 'coherency_sum', swapped)
+    assert _fails(test_point_source_fourier_relation)
+    monkeypatch.setattr(rime_np, 'coherency_sum', good_cs)
+
+    # and the unplanted oracle passes all four
+    test_k_jones_brute_force_and_unit_modulus()
+    test_b_sqrt_matches_scipy_sqrtm_and_invariants()
+    test_rime_brute_force(False, False)
+    test_point_source_fourier_relation()
