@@ -41,6 +41,9 @@ def parse():
     p.add_argument('--no-gradient', action='store_true')
     p.add_argument('--large', action='store_true',
                    help='also run the large single-model workload (about a minute)')
+    p.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                   help='first run a plain copy kernel over MIB MiB (reads and writes '
+                        'MIB MiB), the reference point for HBM byte counters')
     return p.parse_args()
 
 
@@ -52,6 +55,22 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or 'unknown'
+
+
+def copy_calibration(mib):
+    """One elementwise copy (dst = src * 1) of ``mib`` MiB of float32: a
+    kernel of known traffic (mib MiB read, mib MiB written) whose byte
+    counters scale the fused kernel's."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print('# copy calibration: %d MiB read, %d MiB written (torch.mul)' % (mib, mib),
+          file=sys.stderr, flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
 
 
 def time_forward(w, dtype, steps, warmup):
@@ -248,6 +267,8 @@ def main():
     from montblanc_amd import roofline
     from oracle import synth
 
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     w = dict(synth.WORKLOADS[args.workload])
     w['name'] = args.workload
     wl = {k: v for k, v in w.items() if k != 'name'}

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 2
+#define MBX_ABI_VERSION 3
 
 enum {
     MBX_OK = 0,
@@ -175,13 +175,23 @@ typedef struct mbx_outputs {
     void *ekb_vis;             /* optional                                */
 } mbx_outputs;
 
-/* Pair-block plan: the baselines grouped into blocks of pb x qb antenna
+/* Launch plan built on the host from the antenna layout (no reference
+ * counterpart: the reference pairs antennas per baseline in each thread,
+ * RimeSumCoherencies.py:154-157).
+ *
+ * Pair blocks: the baselines grouped into blocks of pb x qb antenna
  * pairs (p0 + i, q0 + j) so that one thread reuses each per-antenna Jones
  * pb (or qb) times. blocks is a DEVICE array of nblocks entries of
  * (2 + pb*qb) int32: p0, q0, then the baseline index of slot i*qb + j or -1.
- * Supported shapes: 1x1, 2x2 and 2x4. A NULL plan (or
- * blocks == NULL) selects the generic kernel, which reads antenna1/2 per
- * time step and so also handles layouts that change with time. */
+ * Supported shapes: 1x1, 2x2 and 2x4. blocks == NULL (or a NULL plan)
+ * selects the generic kernel, which reads antenna1/2 per time step and so
+ * also handles layouts that change with time.
+ *
+ * Point sources on the matrix cores: with baseline_map (a DEVICE copy of
+ * mbx_build_baseline_map's output for the same, time-invariant layout)
+ * and point_gemm != 0, the point sources' coherency sums are computed as
+ * a Hermitian matrix product per (time, channel) for na <= 64, and the
+ * selected kernel adds the other source types. */
 typedef struct mbx_pair_plan {
     const int32_t *blocks;
     int32_t nblocks, pb, qb;
@@ -190,6 +200,8 @@ typedef struct mbx_pair_plan {
      * accumulate; 0 = the same with an automatic producer count;
      * <0 = single-role kernel (every wave does both phases). */
     int32_t producer_waves;
+    const int32_t *baseline_map;
+    int32_t point_gemm;
 } mbx_pair_plan;
 
 /* Host-side plan builder (no device work). antenna1/antenna2 are HOST

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libmontblanc_amd.so')
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 APPLY_WEIGHTS = 1
 RESIDUAL_OUTPUT = 2
 
@@ -19,6 +19,9 @@ ERROR_NAMES = {
     -1: 'MBX_E_NULL', -2: 'MBX_E_DIMS', -3: 'MBX_E_EXTENT',
     -4: 'MBX_E_WORKSPACE', -5: 'MBX_E_UNSUPPORTED', -6: 'MBX_E_NO_DEVICE',
 }
+
+
+E_UNSUPPORTED = -5
 
 
 class MontblancHipError(RuntimeError):
@@ -72,7 +75,8 @@ class mbx_outputs(ctypes.Structure):
 class mbx_pair_plan(ctypes.Structure):
     _fields_ = [('blocks', ctypes.c_void_p), ('nblocks', ctypes.c_int32),
                 ('pb', ctypes.c_int32), ('qb', ctypes.c_int32),
-                ('producer_waves', ctypes.c_int32)]
+                ('producer_waves', ctypes.c_int32), ('baseline_map', ctypes.c_void_p),
+                ('point_gemm', ctypes.c_int32)]
 
 
 GRAD_NAMES = ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape', 'G_term')
@@ -206,7 +210,9 @@ def build_pair_plan(antenna1, antenna2, na, pb, qb):
 
 
 def build_baseline_map(antenna1, antenna2, na):
-    """(na, na) int32 pair -> baseline map for the adjoint (host array)."""
+    """(na, na) int32 pair -> baseline map (host array) for the adjoint
+    and the matrix-core point-source path; raises MontblancHipError
+    (MBX_E_UNSUPPORTED) for layouts that change with time or repeat a pair."""
     import numpy as np
     a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
     a2 = np.ascontiguousarray(antenna2, dtype=np.int32)

@@ -270,9 +270,7 @@ int validate_inputs(const mbx_dims *d, const mbx_inputs *in, const mbx_beam_exte
 
 template <typename T>
 static size_t adjoint_ws_bytes(const mbx_dims &d) {
-    size_t beam = size_t(d.beam_lw.local_size) * d.beam_mh.local_size * d.beam_nud.local_size * 4 *
-                  sizeof(BeamTap<T>);
-    beam = (beam + 255) & ~size_t(255);
+    size_t beam = beam_planes_bytes<T>(d);
     size_t vbar = size_t(d.ntime.local_size) * d.nbl.local_size * d.nchan.local_size * 4 * sizeof(cx<T>);
     return beam + vbar;
 }
@@ -303,12 +301,9 @@ static int adjoint(const mbx_dims *d, const mbx_inputs *in, const void *ekb, con
     if (dext(d->ntime) == 0 || dext(d->nbl) == 0 || dext(d->nchan) == 0)
         return hip_status(hipGetLastError(), "adjoint (empty extents)");
 
-    const uint64_t ncube = uint64_t(d->beam_lw.local_size) * d->beam_mh.local_size *
-                           d->beam_nud.local_size * 4;
-    BeamTap<T> *beam = static_cast<BeamTap<T> *>(ws);
-    size_t beam_bytes = (ncube * sizeof(BeamTap<T>) + 255) & ~size_t(255);
-    cx<T> *vbar = reinterpret_cast<cx<T> *>(static_cast<char *>(ws) + beam_bytes);
-    launch_beam_prepare<T>(static_cast<const cx<T> *>(in->E_beam), beam, ncube, stream);
+    BeamCell<T> *beam = static_cast<BeamCell<T> *>(ws);
+    cx<T> *vbar = reinterpret_cast<cx<T> *>(static_cast<char *>(ws) + beam_planes_bytes<T>(*d));
+    launch_beam_planes<T>(*d, in->E_beam, in->frequency, b, beam, stream);
 
     AdjArgs<T> A;
     RimeArgs<T> &R = A.R;

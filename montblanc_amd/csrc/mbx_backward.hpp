@@ -61,11 +61,11 @@ __device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const dou
     const double vl = fmin(LW1, fmax(0.0, vl_raw)), vm = fmin(MH1, fmax(0.0, vm_raw));
     const bool okl = vl_raw >= 0.0 && vl_raw <= LW1, okm = vm_raw >= 0.0 && vm_raw <= MH1;
     const double gl0 = floor(vl), gm0 = floor(vm);
-    const T ld = T(vl - gl0), md = T(vm - gm0), chd = T(Lch[CH * k + 2]);
-    const uint32_t MH = d.beam_mh.local_size, NUD = d.beam_nud.local_size;
+    const T ld = T(vl - gl0), md = T(vm - gm0);
+    const uint32_t MH = d.beam_mh.local_size;
     const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
     const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const uint32_t ic0 = uint32_t(Lch[CH * k]), ic1 = uint32_t(Lch[CH * k + 1]);
+    const BeamCell<T> *plane = A.beam + uint64_t(c) * d.beam_lw.local_size * MH;
     cx<T> S[4], dSl[4], dSm[4];
     T Am[4], dAl[4], dAm[4];
 #pragma unroll
@@ -73,24 +73,26 @@ __device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const dou
         S[x] = dSl[x] = dSm[x] = {T(0), T(0)};
         Am[x] = dAl[x] = dAm[x] = T(0);
     }
+    // bilinear taps of the channel's plane, and their weights' derivatives
+    // along the l and m grid fractions
 #pragma unroll 1
-    for (int tap = 0; tap < 8; ++tap) {
-        const bool hl = tap & 1, hm = (tap >> 1) & 1, hc = tap >> 2;
-        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0, ic = hc ? ic1 : ic0;
-        const T wl = hl ? ld : T(1) - ld, wm = hm ? md : T(1) - md, wc = hc ? chd : T(1) - chd;
-        const T w = wl * wm * wc;
-        const T wdl = (hl ? T(1) : T(-1)) * wm * wc;
-        const T wdm = wl * (hm ? T(1) : T(-1)) * wc;
-        const BeamTap<T> *e = A.beam + ((uint64_t(il * MH + im) * NUD) + ic) * 4;
+    for (int tap = 0; tap < 4; ++tap) {
+        const bool hl = tap & 1, hm = (tap >> 1) & 1;
+        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0;
+        const T wl = hl ? ld : T(1) - ld, wm = hm ? md : T(1) - md;
+        const T w = wl * wm;
+        const T wdl = hl ? wm : -wm;
+        const T wdm = hm ? wl : -wl;
+        const BeamCell<T> e = plane[il * MH + im];
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
-            const BeamTap<T> v = e[x];
-            S[x].re = fma(w, v.re, S[x].re);     S[x].im = fma(w, v.im, S[x].im);
-            dSl[x].re = fma(wdl, v.re, dSl[x].re); dSl[x].im = fma(wdl, v.im, dSl[x].im);
-            dSm[x].re = fma(wdm, v.re, dSm[x].re); dSm[x].im = fma(wdm, v.im, dSm[x].im);
-            Am[x] = fma(w, v.amp, Am[x]);
-            dAl[x] = fma(wdl, v.amp, dAl[x]);
-            dAm[x] = fma(wdm, v.amp, dAm[x]);
+            const T vre = e.v[3 * x], vim = e.v[3 * x + 1], vamp = e.v[3 * x + 2];
+            S[x].re = fma(w, vre, S[x].re);     S[x].im = fma(w, vim, S[x].im);
+            dSl[x].re = fma(wdl, vre, dSl[x].re); dSl[x].im = fma(wdl, vim, dSl[x].im);
+            dSm[x].re = fma(wdm, vre, dSm[x].re); dSm[x].im = fma(wdm, vim, dSm[x].im);
+            Am[x] = fma(w, vamp, Am[x]);
+            dAl[x] = fma(wdl, vamp, dAl[x]);
+            dAm[x] = fma(wdm, vamp, dAm[x]);
         }
     }
     J4<T> E, dEl, dEm;

@@ -358,8 +358,9 @@ static int plan_batched(const mbx_dims &d, const mbx_batch_tuning *tune, BatPlan
     if (p.lds > 64 * 1024) return set_error(MBX_E_UNSUPPORTED, "batched: LDS tile too large (%zu)", p.lds);
     p.nctile = (d.nchan.local_size + p.ct - 1) / p.ct;
     int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
     p.nwg = (tune && tune->workgroups > 0) ? tune->workgroups : 2 * cus;
     return MBX_OK;
 }
@@ -368,9 +369,7 @@ template <typename T>
 static size_t batched_ws_bytes(const mbx_dims &d, const mbx_batch_tuning *tune) {
     BatPlan p;
     if (plan_batched<T>(d, tune, p)) return 0;
-    size_t beam = size_t(d.beam_lw.local_size) * d.beam_mh.local_size * d.beam_nud.local_size * 4 *
-                  sizeof(BeamTap<T>);
-    beam = (beam + 255) & ~size_t(255);
+    size_t beam = beam_planes_bytes<T>(d);
     size_t slots = size_t(p.nwg) * d.nbl.local_size * p.ct * 4 * sizeof(cx<T>);
     return beam + slots + 256;
 }
@@ -409,11 +408,9 @@ static int batched(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *bt,
     }
     if (dext(d->ntime) == 0 || dext(d->nbl) == 0 || dext(d->nchan) == 0)
         return hip_status(hipGetLastError(), "batched (empty extents)");
-    const uint64_t ncube = uint64_t(d->beam_lw.local_size) * d->beam_mh.local_size *
-                           d->beam_nud.local_size * 4;
-    BeamTap<T> *beam = static_cast<BeamTap<T> *>(ws);
-    size_t beam_bytes = (ncube * sizeof(BeamTap<T>) + 255) & ~size_t(255);
-    launch_beam_prepare<T>(static_cast<const cx<T> *>(in->E_beam), beam, ncube, stream);
+    BeamCell<T> *beam = static_cast<BeamCell<T> *>(ws);
+    const size_t beam_bytes = beam_planes_bytes<T>(*d);
+    launch_beam_planes<T>(*d, in->E_beam, in->frequency, b, beam, stream);
 
     BatchArgs<T> B;
     fill_args<T>(B.R, d, in, b, k, options);

@@ -24,7 +24,7 @@
 
 #include <algorithm>
 
-#include "mbx_tile.hpp"
+#include "mbx_gemm.hpp"
 
 namespace mbx {
 
@@ -32,14 +32,18 @@ void launch_chi2_final(const double *partial, unsigned n, double scale, double *
 
 constexpr int kGenThreads = 256;
 
-// Workgroup index -> (baseline chunk, channel tile, time).
+// Workgroup index -> (time, baseline chunk, channel tile), time fastest:
+// the dispatcher deals consecutive workgroups to the eight XCDs in turn,
+// so the workgroups one XCD runs together share a channel tile and with
+// it the two frequency planes of the beam cube they read (L2 per XCD).
 template <typename T>
 __device__ __forceinline__ void wg_coords(const RimeArgs<T> &A, int &bc, int &ctile, int &t) {
     uint64_t wg = blockIdx.x;
-    bc = wg % A.nbchunk;
-    wg /= A.nbchunk;
-    ctile = wg % A.nctile;
-    t = wg / A.nctile;
+    const uint64_t NT = A.d.ntime.local_size;
+    t = int(wg % NT);
+    wg /= NT;
+    bc = int(wg % A.nbchunk);
+    ctile = int(wg / A.nbchunk);
 }
 
 // ---------------------------------------------------------------------
@@ -82,8 +86,8 @@ k_forward_gen(RimeArgs<T> A) {
             iq[k] = min(max(A.a2[uint64_t(t) * NBL + b], 0), NA - 1);
         }
         acc[k] = jzero<T>();
-        if (ok && d.nsrc.lower_extent > 0 && c0 + icl[k] < nc_ext)
-            acc[k] = jload(A.ekb + ((uint64_t(t) * NBL + b) * NC + c0 + icl[k]) * 4);
+        if (ok && A.vin && c0 + icl[k] < nc_ext)
+            acc[k] = jload(A.vin + ((uint64_t(t) * NBL + b) * NC + c0 + icl[k]) * 4);
     }
     __syncthreads();
 #pragma unroll
@@ -92,7 +96,7 @@ k_forward_gen(RimeArgs<T> A) {
         iv[k] = T(L.uvw[3 * iq[k] + 1] - L.uvw[3 * ip[k] + 1]);
     }
     const int ns_all = dext(d.nsrc);
-    for (int s0 = 0; s0 < ns_all; s0 += st) {
+    for (int s0 = A.s_begin; s0 < ns_all; s0 += st) {
         const int nst = min(st, ns_all - s0);
         if (active) tile_phase0(A, L, t, c0, s0, nst, tid, kGenThreads);
         __syncthreads();
@@ -185,8 +189,8 @@ k_forward_blk(RimeArgs<T> A) {
         for (int j = 0; j < QB; ++j) {
             acc[i][j] = jzero<T>();
             const int b = mine ? plan[2 + i * QB + j] : -1;
-            if (b >= 0 && chan_ok && d.nsrc.lower_extent > 0)
-                acc[i][j] = jload(A.ekb + ((uint64_t(t) * NBL + b) * NC + c0 + k) * 4);
+            if (b >= 0 && chan_ok && A.vin)
+                acc[i][j] = jload(A.vin + ((uint64_t(t) * NBL + b) * NC + c0 + k) * 4);
         }
     __syncthreads();
     T up[PB], vp[PB], uq[QB], vq[QB];
@@ -196,7 +200,7 @@ k_forward_blk(RimeArgs<T> A) {
     for (int j = 0; j < QB; ++j) { uq[j] = T(L.uvw[3 * qj[j]]); vq[j] = T(L.uvw[3 * qj[j] + 1]); }
 
     const int ns_all = dext(d.nsrc);
-    for (int s0 = 0; s0 < ns_all; s0 += st) {
+    for (int s0 = A.s_begin; s0 < ns_all; s0 += st) {
         const int nst = min(st, ns_all - s0);
         if (active) tile_phase0(A, L, t, c0, s0, nst, tid, nth);
         __syncthreads();
@@ -325,8 +329,8 @@ k_forward_pc(RimeArgs<T> A, int cth) {
     const int c0 = ctile * ct;
     if (active) tile_setup<T>(A, L.uvw, L.pa, L.ch, t, c0, tid, nth);
     __syncthreads();
-    const int ns_all = dext(d.nsrc);
-    const int ntiles = (ns_all + st - 1) / st;
+    const int ns_all = dext(d.nsrc), sb = A.s_begin;
+    const int ntiles = (ns_all - sb + st - 1) / st;
     // The role is a per-wave (scalar) decision, so each wave runs exactly
     // one of the two loops below; both execute 1 + ntiles barriers.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -355,8 +359,8 @@ k_forward_pc(RimeArgs<T> A, int cth) {
             for (int j = 0; j < QB; ++j) {
                 acc[i][j] = jzero<T>();
                 const int b = mine ? plan[2 + i * QB + j] : -1;
-                if (b >= 0 && chan_ok && d.nsrc.lower_extent > 0)
-                    acc[i][j] = jload(A.ekb + ((uint64_t(t) * NBL + b) * NC + c0 + k) * 4);
+                if (b >= 0 && chan_ok && A.vin)
+                    acc[i][j] = jload(A.vin + ((uint64_t(t) * NBL + b) * NC + c0 + k) * 4);
             }
         T up[PB], vp[PB], uq[QB], vq[QB];
 #pragma unroll
@@ -365,7 +369,7 @@ k_forward_pc(RimeArgs<T> A, int cth) {
         for (int j = 0; j < QB; ++j) { uq[j] = T(L.uvw[3 * qj[j]]); vq[j] = T(L.uvw[3 * qj[j] + 1]); }
         __syncthreads();                       // tile 0 produced
         for (int it = 0; it < ntiles; ++it) {
-            const int nst = min(st, ns_all - it * st);
+            const int nst = min(st, ns_all - sb - it * st);
             if (mine) pc_consume<T, PB, QB>(A, L, it & 1, nst, k, pi, qj, up, vp, uq, vq, acc);
             __syncthreads();
         }
@@ -384,10 +388,10 @@ k_forward_pc(RimeArgs<T> A, int cth) {
         }
     } else {
         const int ptid = tid - cth, pth = nth - cth;
-        if (active && ntiles > 0) pc_produce<T>(A, L, 0, t, c0, 0, min(st, ns_all), ptid, pth);
+        if (active && ntiles > 0) pc_produce<T>(A, L, 0, t, c0, sb, min(st, ns_all - sb), ptid, pth);
         __syncthreads();
         for (int it = 0; it < ntiles; ++it) {
-            const int s1 = (it + 1) * st;
+            const int s1 = sb + (it + 1) * st;
             if (active && it + 1 < ntiles)
                 pc_produce<T>(A, L, (it + 1) & 1, t, c0, s1, min(st, ns_all - s1), ptid, pth);
             __syncthreads();
@@ -485,9 +489,7 @@ static uint64_t max_partials(const mbx_dims &d) {
 
 template <typename T>
 static size_t beam_ws_bytes(const mbx_dims &d) {
-    size_t beam = size_t(d.beam_lw.local_size) * d.beam_mh.local_size * d.beam_nud.local_size * 4 *
-                  sizeof(BeamTap<T>);
-    return (beam + 255) & ~size_t(255);
+    return beam_planes_bytes<T>(d);
 }
 
 template <typename T>
@@ -568,22 +570,31 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
         return set_error(MBX_E_WORKSPACE, "forward: workspace needs %zu bytes (got %zu)", need, ws_bytes);
 
     FwdPlan p = plan_forward<T>(*d, pp);
-    const uint64_t ncube = uint64_t(d->beam_lw.local_size) * d->beam_mh.local_size *
-                           d->beam_nud.local_size * 4;
-    BeamTap<T> *beam = static_cast<BeamTap<T> *>(ws);
+    BeamCell<T> *beam = static_cast<BeamCell<T> *>(ws);
     double *partial = reinterpret_cast<double *>(static_cast<char *>(ws) + beam_ws_bytes<T>(*d));
     const double scale = (options & MBX_APPLY_WEIGHTS) ? 1.0 : 1.0 / k->sigma_sqrd;
     if (dext(d->ntime) == 0 || dext(d->nbl) == 0 || dext(d->nchan) == 0) {
         if (final_batch) launch_chi2_final(partial, 0, scale, out->X2, stream);
         return hip_status(hipGetLastError(), "forward (empty extents)");
     }
-    launch_beam_prepare<T>(static_cast<const cx<T> *>(in->E_beam), beam, ncube, stream);
+    launch_beam_planes<T>(*d, in->E_beam, in->frequency, b, beam, stream);
 
     RimeArgs<T> A;
     fill_args(A, d, in, b, k, options);
     A.beam = beam;
     A.model = (cx<T> *)out->model_vis; A.chi2 = (T *)out->chi_sqrd_result;
     A.ekb = (cx<T> *)out->ekb_vis; A.partial = partial;
+    // running sum of earlier source batches (RimeSumCoherencies.py:183-190)
+    A.vin = d->nsrc.lower_extent > 0 ? A.ekb : nullptr;
+    if (pp && pp->baseline_map && pp->point_gemm && point_gemm_supported<T>(*d)) {
+        // point sources on the matrix cores into the running-sum buffer,
+        // which the kernel below continues from
+        cx<T> *vbuf = A.ekb ? A.ekb : A.model;
+        r = launch_point_gemm<T>(A, pp->baseline_map, vbuf, A.vin, stream);
+        if (r) return r;
+        A.vin = vbuf;
+        A.s_begin = dext(d->npsrc);
+    }
     A.ct = p.ct; A.st = p.st; A.nbchunk = p.nbchunk; A.bchunk = p.bchunk; A.nctile = p.nctile;
     if (p.blocked) {
         A.blocks = pp->blocks; A.nblocks = pp->nblocks; A.pb = pp->pb; A.qb = pp->qb;

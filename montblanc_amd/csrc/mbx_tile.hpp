@@ -15,44 +15,84 @@
 
 namespace mbx {
 
-// Prepared beam cube entry: value and amplitude of one (l, m, nu, corr).
-template <typename T> struct BeamTap { T re, im, amp, pad; };
+// Prepared beam: the cube interpolated along frequency to each channel
+// once per launch (the frequency weight of the trilinear lookup depends on
+// the channel only), one cell per (channel, l, m) holding (re, im, |.|)
+// of the four correlations. A Jones lookup then gathers four 48/96-byte
+// cells instead of eight taps of four correlations.
+template <typename T> struct alignas(4 * sizeof(T)) BeamCell { T v[12]; };
 
 template <typename T> struct RimeArgs {
     mbx_dims d;
     const T *uvw, *lm, *stokes, *alpha, *freq, *ref_freq, *pa, *pe, *ascale, *gauss, *sersic, *wv;
-    const BeamTap<T> *beam;
+    const BeamCell<T> *beam;  // [nchan][beam_lw][beam_mh]
     const int32_t *a1, *a2;
     const uint8_t *flag;
     const cx<T> *obs, *G;
     cx<T> *model;
     T *chi2;
     cx<T> *ekb;
+    const cx<T> *vin;         // running source sum to start from (or NULL)
     double *partial;
     double ll, lmin, lscale, mscale, lfreq, fscale;
     double gauss_scale, two_pi_over_c;
     int options;
     int ct, st, nbchunk, bchunk, nctile;
+    int s_begin;              // first source the kernel sums (earlier ones are in vin)
     const int32_t *blocks;    // pair-block plan (blocked kernels)
     int nblocks, pb, qb;
 };
 
+// Cell (c, l, m): (1 - w) X[l, m, g0] + w X[l, m, g1] for X = value and
+// |value| of each correlation, with g0, g1, w the channel's grid position
+// (RimeEBeam.py:233-260: nu = (nud-1)(f - lfreq)/(ufreq - lfreq) clamped
+// to [0, nud-1], g1 = min(g0 + 1, nud-1)).
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_beam_prepare(const cx<T> *__restrict__ E, BeamTap<T> *__restrict__ out, uint64_t n) {
+k_beam_planes(const cx<T> *__restrict__ E, BeamCell<T> *__restrict__ out,
+              const T *__restrict__ freq, uint32_t nchan, uint32_t lw, uint32_t mh, uint32_t nud,
+              double lfreq, double fscale) {
+    const uint64_t n = uint64_t(nchan) * lw * mh;
+    const double NUD1 = double(nud - 1);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        cx<T> e = E[i];
-        out[i] = {e.re, e.im, sqrt(e.re * e.re + e.im * e.im), T(0)};
+        const uint64_t lm = i % (uint64_t(lw) * mh);
+        const uint32_t c = uint32_t(i / (uint64_t(lw) * mh));
+        const double vc = fmin(NUD1, fmax(0.0, fscale * (double(freq[c]) - lfreq)));
+        const double g0 = floor(vc);
+        const uint32_t ic0 = uint32_t(g0), ic1 = uint32_t(fmin(g0 + 1.0, NUD1));
+        const T w1 = T(vc - g0), w0 = T(1) - w1;
+        const cx<T> *e0 = E + (lm * nud + ic0) * 4, *e1 = E + (lm * nud + ic1) * 4;
+        BeamCell<T> cell;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            const cx<T> a = e0[x], b = e1[x];
+            cell.v[3 * x] = w0 * a.re + w1 * b.re;
+            cell.v[3 * x + 1] = w0 * a.im + w1 * b.im;
+            cell.v[3 * x + 2] = w0 * sqrt(a.re * a.re + a.im * a.im) + w1 * sqrt(b.re * b.re + b.im * b.im);
+        }
+        out[i] = cell;
     }
 }
 
-// Value and amplitude of every beam-cube entry, once per launch.
 template <typename T>
-static inline void launch_beam_prepare(const cx<T> *E, BeamTap<T> *beam, uint64_t n,
-                                       hipStream_t stream) {
-    unsigned g = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
-    hipLaunchKernelGGL(k_beam_prepare<T>, dim3(g), dim3(256), 0, stream, E, beam, n);
+static inline size_t beam_planes_bytes(const mbx_dims &d) {
+    const size_t n = size_t(d.nchan.local_size) * d.beam_lw.local_size * d.beam_mh.local_size;
+    return (n * sizeof(BeamCell<T>) + 255) & ~size_t(255);
+}
+
+// Channel-interpolated beam planes of every channel, once per launch.
+template <typename T>
+static inline void launch_beam_planes(const mbx_dims &d, const void *E_beam, const void *freq,
+                                      const mbx_beam_extents *b, BeamCell<T> *out,
+                                      hipStream_t stream) {
+    const uint64_t n = uint64_t(d.nchan.local_size) * d.beam_lw.local_size * d.beam_mh.local_size;
+    const unsigned g = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    const double fscale = double(d.beam_nud.local_size - 1) / (b->ufreq - b->lfreq);
+    hipLaunchKernelGGL(k_beam_planes<T>, dim3(g), dim3(256), 0, stream,
+                       static_cast<const cx<T> *>(E_beam), out, static_cast<const T *>(freq),
+                       d.nchan.local_size, d.beam_lw.local_size, d.beam_mh.local_size,
+                       d.beam_nud.local_size, b->lfreq, fscale);
 }
 
 // Argument block from the C-ABI structs (beam cube pointer, outputs and
@@ -76,7 +116,8 @@ static inline void fill_args(RimeArgs<T> &A, const mbx_dims *d, const mbx_inputs
     A.fscale = double(d->beam_nud.local_size - 1) / (b->ufreq - b->lfreq);
     A.gauss_scale = k->gauss_scale; A.two_pi_over_c = k->two_pi_over_c;
     A.options = options;
-    A.model = nullptr; A.chi2 = nullptr; A.ekb = nullptr; A.partial = nullptr;
+    A.model = nullptr; A.chi2 = nullptr; A.ekb = nullptr; A.vin = nullptr; A.partial = nullptr;
+    A.s_begin = 0;
     A.blocks = nullptr; A.nblocks = 0; A.pb = A.qb = 0;
     A.ct = A.st = A.nbchunk = A.bchunk = A.nctile = 1;
 }
@@ -269,49 +310,48 @@ __device__ __forceinline__ void tile_phase0(const RimeArgs<T> &A, Tile<T> &L, in
     }
 }
 
-// Beam Jones of one (source, antenna, channel) from the prepared cube
-// (RimeEBeam.py:233-343 / CPUSolver.py:348-475 semantics); grid
-// coordinates in float64.
-template <typename T>
-__device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const double *Lpa,
-                                            const double *Lch, int t, int a, int c, int k,
-                                            double l0, double m0) {
+// Beam Jones from the prepared planes (RimeEBeam.py:233-343 /
+// CPUSolver.py:348-475 semantics: trilinear interpolation of the values
+// and, separately, of their amplitudes, then E = value * amp / |value|)
+// given the antenna's rotation (sin, cos of the parallactic angle),
+// pointing error and scaling for this (time, channel c). Grid coordinates
+// in float64. UT of the four bilinear taps are loaded together.
+template <typename T, int UT = 1>
+__device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, double pc,
+                                               double pel, double pem, double asl, double asm_,
+                                               int c, double l0, double m0) {
     const mbx_dims &d = A.d;
-    const uint64_t NA = d.na.local_size, NC = d.nchan.local_size;
-    const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
-    const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
-    const double ps = Lpa[2 * a], pc = Lpa[2 * a + 1];
-    double l = (l0 * pc - m0 * ps + double(pet[0])) * double(asc[0]);
-    double m = (l0 * ps + m0 * pc + double(pet[1])) * double(asc[1]);
+    double l = (l0 * pc - m0 * ps + pel) * asl;
+    double m = (l0 * ps + m0 * pc + pem) * asm_;
     const double LW1 = double(d.beam_lw.local_size - 1), MH1 = double(d.beam_mh.local_size - 1);
     double vl = fmin(LW1, fmax(0.0, A.lscale * (l - A.ll)));
     double gl0 = floor(vl);
     double vm = fmin(MH1, fmax(0.0, A.mscale * (m - A.lmin)));
     double gm0 = floor(vm);
-    const T ld = T(vl - gl0), md = T(vm - gm0), chd = T(Lch[CH * k + 2]);
-    const uint32_t MH = d.beam_mh.local_size, NUD = d.beam_nud.local_size;
+    const T ld = T(vl - gl0), md = T(vm - gm0);
+    const uint32_t MH = d.beam_mh.local_size;
     const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
     const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const uint32_t ic0 = uint32_t(Lch[CH * k]), ic1 = uint32_t(Lch[CH * k + 1]);
-    const T wc0 = T(1) - chd, wc1 = chd;
+    const BeamCell<T> *plane = A.beam + uint64_t(c) * d.beam_lw.local_size * MH;
     cx<T> acc[4];
     T amp[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) { acc[p] = {T(0), T(0)}; amp[p] = T(0); }
-    // The eight taps one at a time: bounds the registers held by loads in
-    // flight (the kernels hide the latency with other waves).
+    static_assert(UT == 1 || UT == 2 || UT == 4, "taps per step");
 #pragma unroll 1
-    for (int tap = 0; tap < 8; ++tap) {
-        const bool hl = tap & 1, hm = (tap >> 1) & 1, hc = tap >> 2;
-        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0, ic = hc ? ic1 : ic0;
-        const T w = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md) * (hc ? wc1 : wc0);
-        const BeamTap<T> *e = A.beam + ((uint64_t(il * MH + im) * NUD) + ic) * 4;
+    for (int tap0 = 0; tap0 < 4; tap0 += UT)
+#pragma unroll
+    for (int u = 0; u < UT; ++u) {
+        const int tap = tap0 + u;
+        const bool hl = tap & 1, hm = (tap >> 1) & 1;
+        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0;
+        const T w = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md);
+        const BeamCell<T> e = plane[il * MH + im];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            const BeamTap<T> v = e[p];
-            acc[p].re = fma(w, v.re, acc[p].re);
-            acc[p].im = fma(w, v.im, acc[p].im);
-            amp[p] = fma(w, v.amp, amp[p]);
+            acc[p].re = fma(w, e.v[3 * p], acc[p].re);
+            acc[p].im = fma(w, e.v[3 * p + 1], acc[p].im);
+            amp[p] = fma(w, e.v[3 * p + 2], amp[p]);
         }
     }
     J4<T> E;
@@ -322,6 +362,21 @@ __device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const double *
         E.v[p] = {acc[p].re * g, acc[p].im * g};
     }
     return E;
+}
+
+// Beam Jones of one (source, antenna, channel), reading the antenna's
+// pointing error and scaling from global memory.
+template <typename T, int UT = 1>
+__device__ __forceinline__ J4<T> beam_jones(const RimeArgs<T> &A, const double *Lpa,
+                                            const double *Lch, int t, int a, int c, int k,
+                                            double l0, double m0) {
+    const mbx_dims &d = A.d;
+    const uint64_t NA = d.na.local_size, NC = d.nchan.local_size;
+    const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
+    const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
+    (void)Lch; (void)k;
+    return beam_jones_at<T, UT>(A, Lpa[2 * a], Lpa[2 * a + 1], double(pet[0]), double(pet[1]),
+                                double(asc[0]), double(asc[1]), c, l0, m0);
 }
 
 // Phase 1: J = E . (K sqrt(B)) for every (source, antenna, channel) of
@@ -361,7 +416,7 @@ template <> __device__ __forceinline__ double exp_t<double>(double x) { return e
 // Per-antenna Jones of one (source s, antenna a, channel c = c0 + k) with
 // every source quantity read from global memory (producer waves of the
 // double-buffered kernels, which share no per-source LDS stage).
-template <typename T>
+template <typename T, int UT = 1>
 __device__ __forceinline__ J4<T> antenna_jones(const RimeArgs<T> &A, const double *Luvw,
                                                const double *Lpa, const double *Lch, int t, int s,
                                                int a, int c, int k) {
@@ -371,7 +426,7 @@ __device__ __forceinline__ J4<T> antenna_jones(const RimeArgs<T> &A, const doubl
     const T *S = A.stokes + (uint64_t(s) * NT + t) * 4;
     const T power = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * Lch[CH * k + 4]);
     const J4<T> B = brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
-    const J4<T> E = beam_jones<T>(A, Lpa, Lch, t, a, c, k, l0, m0);
+    const J4<T> E = beam_jones<T, UT>(A, Lpa, Lch, t, a, c, k, l0, m0);
     const double *x = Luvw + 3 * a;
     const double phi = -A.two_pi_over_c * Lch[CH * k + 3] * (x[2] * nm1 + x[1] * m0 + x[0] * l0);
     T sn, cs;

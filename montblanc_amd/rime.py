@@ -152,8 +152,45 @@ def alloc_outputs(arrays, keep_ekb=False):
 DEFAULT_PAIR_BLOCK = {'f32': (2, 2, 0), 'f64': 'generic'}
 
 
-def pair_plan(arrays, pair_block=None):
-    """Device pair-block plan for the antenna layout of ``arrays`` (cached in
+def build_plan(antenna1, antenna2, na, device, form, point_gemm=True):
+    """Launch plan from HOST antenna arrays: (mbx_pair_plan, device buffers
+    it points to), or None for the generic kernel without the matrix-core
+    point-source path.
+
+    form: (pb, qb[, producer_waves]) pair blocks, or 'generic'.
+    point_gemm: use the matrix cores for point sources when the layout is
+    time-invariant with one baseline per antenna pair.
+    """
+    keep = []
+    blocks, nblocks, pb, qb, pw = None, 0, 0, 0, 0
+    if form != 'generic':
+        f = tuple(form)
+        pb, qb = f[0], f[1]
+        pw = f[2] if len(f) > 2 else 0
+        host = _lib.build_pair_plan(antenna1, antenna2, na, pb, qb)
+        if host is not None:
+            dev = torch.from_numpy(host).to(device)
+            keep.append(dev)
+            blocks, nblocks = dev.data_ptr(), host.shape[0]
+    bmap = None
+    if point_gemm:
+        try:
+            hm = _lib.build_baseline_map(antenna1, antenna2, na)
+        except _lib.MontblancHipError as e:
+            if e.code != _lib.E_UNSUPPORTED:
+                raise
+            hm = None
+        if hm is not None:
+            dm = torch.from_numpy(hm).to(device)
+            keep.append(dm)
+            bmap = dm.data_ptr()
+    if blocks is None and bmap is None:
+        return None
+    return (_lib.mbx_pair_plan(blocks, nblocks, pb, qb, pw, bmap, 1 if bmap else 0), keep)
+
+
+def pair_plan(arrays, pair_block=None, point_gemm=True):
+    """Device launch plan for the antenna layout of ``arrays`` (cached in
     the dict under '_pair_plan'); None selects the generic kernel.
 
     pair_block: (pb, qb[, producer_waves]), 'generic', or None for the
@@ -161,34 +198,27 @@ def pair_plan(arrays, pair_block=None):
     """
     if pair_block is None:
         pair_block = DEFAULT_PAIR_BLOCK[suffix(arrays)]
-    if pair_block == 'generic':
-        return None
-    pbq = tuple(pair_block)
-    pb, qb = pbq[0], pbq[1]
-    pw = pbq[2] if len(pbq) > 2 else 0
-    key = ('_pair_plan', pb, qb, pw)
+    key = ('_pair_plan', pair_block if pair_block == 'generic' else tuple(pair_block),
+           bool(point_gemm))
     cached = arrays.get('_pair_plan')
     if cached is not None and cached[0] == key:
         return cached[1]
-    na = arrays['uvw'].shape[1]
-    host = _lib.build_pair_plan(arrays['antenna1'].cpu().numpy(), arrays['antenna2'].cpu().numpy(),
-                                na, pb, qb)
-    plan = None
-    if host is not None:
-        dev = torch.from_numpy(host).to(arrays['uvw'].device)
-        plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, pw), dev)
+    plan = build_plan(arrays['antenna1'].cpu().numpy(), arrays['antenna2'].cpu().numpy(),
+                      arrays['uvw'].shape[1], arrays['uvw'].device, pair_block, point_gemm)
     arrays['_pair_plan'] = (key, plan)
     return plan
 
 
 def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=None,
-            outputs=None, keep_ekb=False, dims=None, stream=None, pair_block=None):
+            outputs=None, keep_ekb=False, dims=None, stream=None, pair_block=None,
+            point_gemm=True):
     """Fused forward on device tensors. Returns the outputs dict
-    (model_vis, chi_sqrd_result, X2 [device float64 scalar], ekb_vis?)."""
+    (model_vis, chi_sqrd_result, X2 [device float64 scalar], ekb_vis?).
+    point_gemm=False keeps the point sources on the vector units."""
     dev = arrays['uvw'].device
     out = outputs if outputs is not None else alloc_outputs(arrays, keep_ekb)
     d = make_dims(arrays, counts) if dims is None else dims
-    plan = pair_plan(arrays, pair_block)
+    plan = pair_plan(arrays, pair_block, point_gemm)
     ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
     outs = _lib.mbx_outputs(_raw(out.get('model_vis')), _raw(out.get('chi_sqrd_result')),
                             _raw(out.get('X2')), _raw(out.get('ekb_vis')))

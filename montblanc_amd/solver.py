@@ -358,24 +358,16 @@ class RimeSolver(object):
             setattr(self, 'retrieve_' + name, (lambda _n=name: self.retrieve(_n)))
 
     def pair_plan(self):
-        """Device pair-block plan of the current antenna layout (rebuilt after
-        antenna1/antenna2 transfers); None when the layout varies with time."""
+        """Device launch plan of the current antenna layout (rebuilt after
+        antenna1/antenna2 transfers); None for the generic kernel."""
         if self._pair_plan is None:
-            from .rime import DEFAULT_PAIR_BLOCK
+            from .rime import DEFAULT_PAIR_BLOCK, build_plan
             form = DEFAULT_PAIR_BLOCK['f32' if self.is_float() else 'f64']
-            if form == 'generic':
-                self._pair_plan = False
-                return None
-            pb, qb, pw = form
             for i, name in enumerate(('antenna1', 'antenna2')):
                 if self._host_antenna[i] is None:
                     self._host_antenna[i] = self.retrieve(name)
-            host = _lib.build_pair_plan(self._host_antenna[0], self._host_antenna[1],
-                                        self.dim_local_size('na'), pb, qb)
-            plan = None
-            if host is not None:
-                dev = torch.from_numpy(host).to(self.device)
-                plan = (_lib.mbx_pair_plan(dev.data_ptr(), host.shape[0], pb, qb, pw), dev)
+            plan = build_plan(self._host_antenna[0], self._host_antenna[1],
+                              self.dim_local_size('na'), self.device, form)
             self._pair_plan = plan if plan is not None else False
         return self._pair_plan or None
 
