@@ -41,6 +41,7 @@ def parse():
     p.add_argument('--no-gradient', action='store_true')
     p.add_argument('--large', action='store_true',
                    help='also run the large single-model workload (about a minute)')
+    p.add_argument('--no-batched', action='store_true')
     p.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                    help='first run a plain copy kernel over MIB MiB (reads and writes '
                         'MIB MiB), the reference point for HBM byte counters')
@@ -237,6 +238,57 @@ def time_large(steps=1):
             'error_slice': 'time 0, channels 0-1, all 2016 baselines and 12000 sources'}
 
 
+def time_batched(steps=2):
+    """Batched workload: 256 sky-model samples, source counts log-uniform
+    in [10, 2000] (30% gaussian), against one shared observation of 64
+    antennas, 50 times, 32 channels; float32 with float64 phase. chi^2 of
+    every sample, then chi^2 + gradient of every sample, each in one
+    persistent launch with autotuned tile shapes (the tuning runs in the
+    untimed warm-up)."""
+    import numpy as np
+    import torch
+    from montblanc_amd import batched, roofline
+    from oracle import synth
+
+    w = dict(synth.WORKLOADS['batched_64ant'])
+    rng = np.random.default_rng(1003)
+    counts = batched.sample_counts(rng, 256)
+    shared = synth.make_inputs(**w, seed=1003)
+    nt = w['ntime']
+    samples = []
+    for c in counts:
+        n = sum(c)
+        I = rng.lognormal(0.0, 1.0, n)
+        st = np.empty((n, nt, 4))
+        st[..., 0] = I[:, None]
+        st[..., 1:] = (I[:, None] * rng.uniform(-0.1, 0.1, (n, 3)))[:, None, :]
+        samples.append(dict(lm=rng.uniform(-0.01, 0.01, (n, 2)), stokes=st,
+                            alpha=np.repeat(rng.normal(-0.7, 0.2, n)[:, None], nt, axis=1),
+                            gauss_shape=rng.uniform(-1e-4, 1e-4, (3, c[1])),
+                            sersic_shape=np.zeros((3, 0)), counts=c))
+    b = batched.Batch(shared, samples, dtype=np.float32)
+    nbl = w['na'] * (w['na'] - 1) // 2
+    n_terms = sum(sum(c) for c in counts) * nt * nbl * w['nchan']
+    out = {}
+    for label, grads in (('chi2', False), ('chi2_gradient', True)):
+        batched.evaluate(b, gradients=grads)            # tunes and warms up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batched.evaluate(b, gradients=grads)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[label] = {'s_per_step': dt, 'terms_per_s': n_terms / dt}
+        print('# batched %s: %.1f ms, %.4g terms/s' % (label, dt * 1e3, n_terms / dt),
+              file=sys.stderr, flush=True)
+    out.update(workload='batched_64ant', samples=len(counts), sources=sum(sum(c) for c in counts),
+               terms=n_terms, dtype='f32',
+               gradient_over_chi2=out['chi2']['s_per_step'] / out['chi2_gradient']['s_per_step'])
+    del b
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(w):
     """numpy restatement of the reference CPU chain on four time steps and
     16 channels, all sources and baselines (~10-30 s on one core)."""
@@ -314,6 +366,8 @@ def main():
               % (grad['forward_ms'], grad['forward_gradient_ms'], grad['gradient_over_forward']),
               file=sys.stderr, flush=True)
         line['gradient'] = grad
+    if not args.no_batched:
+        line['batched'] = time_batched()
     if args.large:
         line['large'] = time_large()
         print('# large: %s' % line['large'], file=sys.stderr, flush=True)

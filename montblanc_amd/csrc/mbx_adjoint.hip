@@ -25,19 +25,11 @@
 #include "mbx_backward.hpp"
 
 namespace mbx {
+template <typename T> bool adjoint_gemm_supported(const mbx_dims &d);
+template <typename T> int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream);
+}  // namespace mbx
 
-template <typename T> struct AdjArgs {
-    RimeArgs<T> R;
-    const cx<T> *ekb;        // forward's pre-gain sums
-    cx<T> *vbar;             // workspace: Vb per visibility
-    const int32_t *bmap;     // (na x na) baseline map
-    double *g_lm, *g_stokes, *g_alpha, *g_gauss, *g_sersic;
-    double *g_G;             // (ntime, na, nchan, 4) complex128 as re, im pairs
-    double gscale;           // upstream d L / d chi^2
-    int ct2, nctile2;        // channel tiling of k_adj_vis
-};
-
-constexpr int kBlMask = (1 << 29) - 1;
+namespace mbx {
 
 // ---------------------------------------------------------------------
 // Visibility stage: Mb, Vb and the gain adjoint.
@@ -327,7 +319,10 @@ static int adjoint(const mbx_dims *d, const mbx_inputs *in, const void *ekb, con
     r = hip_status(hipGetLastError(), "adjoint vis launch");
     if (r) return r;
 
-    // source stage: one thread per (antenna, channel) of the tile
+    // source stage: on the matrix cores with the visibility adjoint staged
+    // in LDS where it fits (na <= 64), else one thread per (antenna,
+    // channel) of a tile
+    if (adjoint_gemm_supported<T>(*d)) return launch_adjoint_gemm<T>(A, stream);
     constexpr int STB = adj_stb<T>();
     int ct = std::max(1, std::min<int>(512 / na, d->nchan.local_size));
     auto lds_of = [&](int c) {

@@ -9,6 +9,20 @@
 
 namespace mbx {
 
+template <typename T> struct AdjArgs {
+    RimeArgs<T> R;
+    const cx<T> *ekb;        // forward's pre-gain sums
+    cx<T> *vbar;             // workspace: Vb per visibility
+    const int32_t *bmap;     // (na x na) baseline map
+    double *g_lm, *g_stokes, *g_alpha, *g_gauss, *g_sersic;
+    double *g_G;             // (ntime, na, nchan, 4) complex128 as re, im pairs
+    double gscale;           // upstream d L / d chi^2
+    int ct2, nctile2;        // channel tiling of k_adj_vis
+};
+
+constexpr int kBlMask = (1 << 29) - 1;
+
+
 template <typename T>
 __device__ __forceinline__ J4<T> jherm(const J4<T> &A) {
     J4<T> r;
@@ -39,23 +53,25 @@ __device__ __forceinline__ T jdot_re(const J4<T> &A, const J4<T> &B) {
 // ---------------------------------------------------------------------
 // Per-item backward of J = E K sqrt(B) onto (l, m, I, Q, U, V, alpha).
 
+// Inputs: the source's l, m; its Stokes I, Q, U, V and spectral power
+// P = (f/f_ref)^alpha at this (time, channel); the antenna's uvw (xa),
+// parallactic sin/cos (ps, pc), pointing error (pel, pem) and scaling
+// (al, am); the channel c (beam plane), its frequency f and
+// lnr = log(f/f_ref); and Jb. Writes d chi^2 / d(l, m, I, Q, U, V, alpha)
+// of this item to g.
 template <typename T>
-__device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const double *Luvw,
-                                                 const double *Lpa, const double *Lch, int t,
-                                                 int s, int a, int c, int k, const J4<T> &Jb,
-                                                 double *g /* [7] */) {
+__device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const double *xa,
+                                                    double ps, double pc, double pel, double pem,
+                                                    double al, double am, int c, double f,
+                                                    double lnr, double l0, double m0, T I, T Q,
+                                                    T U, T V, T P, const J4<T> &Jb,
+                                                    double *g /* [7] */) {
     const mbx_dims &d = A.d;
-    const uint64_t NT = d.ntime.local_size, NA = d.na.local_size, NC = d.nchan.local_size;
-    const double l0 = A.lm[2 * uint64_t(s)], m0 = A.lm[2 * uint64_t(s) + 1];
     const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
     const double nm1 = n - 1.0;
     // ---- beam value and its derivatives along the grid fractions
-    const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
-    const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
-    const double ps = Lpa[2 * a], pc = Lpa[2 * a + 1];
-    const double al = double(asc[0]), am = double(asc[1]);
-    const double lr = (l0 * pc - m0 * ps + double(pet[0])) * al;
-    const double mr = (l0 * ps + m0 * pc + double(pet[1])) * am;
+    const double lr = (l0 * pc - m0 * ps + pel) * al;
+    const double mr = (l0 * ps + m0 * pc + pem) * am;
     const double LW1 = double(d.beam_lw.local_size - 1), MH1 = double(d.beam_mh.local_size - 1);
     const double vl_raw = A.lscale * (lr - A.ll), vm_raw = A.mscale * (mr - A.lmin);
     const double vl = fmin(LW1, fmax(0.0, vl_raw)), vm = fmin(MH1, fmax(0.0, vm_raw));
@@ -116,16 +132,10 @@ __device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const dou
         }
     }
     // ---- phase and sqrt(B)
-    const double *xa = Luvw + 3 * a;
-    const double f = Lch[CH * k + 3];
     const double phi = -A.two_pi_over_c * f * (xa[2] * nm1 + xa[1] * m0 + xa[0] * l0);
     T sn, cs;
     PhaseCS<T>::eval(phi, &sn, &cs);
     const cx<T> K = {cs, sn};
-    const T *St = A.stokes + (uint64_t(s) * NT + t) * 4;
-    const T I = St[0], Q = St[1], U = St[2], V = St[3];
-    const double lnr = Lch[CH * k + 4];
-    const T P = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * lnr);
     const T rP = sqrt(P);
     const T s0 = sqrt(I * I - Q * Q - U * U - V * V);
     const T t0 = sqrt(T(2) * I + T(2) * s0);
@@ -178,6 +188,26 @@ __device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const dou
     const double dvm_dl = okm ? A.mscale * am * ps : 0.0, dvm_dm = okm ? A.mscale * am * pc : 0.0;
     g[0] = double(g_phi) * dphi_dl + double(g_ld) * dvl_dl + double(g_md) * dvm_dl;
     g[1] = double(g_phi) * dphi_dm + double(g_ld) * dvl_dm + double(g_md) * dvm_dm;
+}
+
+// The same item reading its inputs from global memory and the tile's
+// per-antenna (Luvw, Lpa) and channel (Lch, record k) LDS arrays.
+template <typename T>
+__device__ __forceinline__ void antenna_backward(const RimeArgs<T> &A, const double *Luvw,
+                                                 const double *Lpa, const double *Lch, int t,
+                                                 int s, int a, int c, int k, const J4<T> &Jb,
+                                                 double *g /* [7] */) {
+    const mbx_dims &d = A.d;
+    const uint64_t NT = d.ntime.local_size, NA = d.na.local_size, NC = d.nchan.local_size;
+    const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
+    const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
+    const T *St = A.stokes + (uint64_t(s) * NT + t) * 4;
+    const double lnr = Lch[CH * k + 4];
+    const T P = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * lnr);
+    antenna_backward_at<T>(A, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], double(pet[0]), double(pet[1]),
+                           double(asc[0]), double(asc[1]), c, Lch[CH * k + 3], lnr,
+                           A.lm[2 * uint64_t(s)], A.lm[2 * uint64_t(s) + 1], St[0], St[1], St[2], St[3],
+                           P, Jb, g);
 }
 
 // Derivatives of the shape factor f of one pair with respect to the three

@@ -586,14 +586,17 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
     A.ekb = (cx<T> *)out->ekb_vis; A.partial = partial;
     // running sum of earlier source batches (RimeSumCoherencies.py:183-190)
     A.vin = d->nsrc.lower_extent > 0 ? A.ekb : nullptr;
-    if (pp && pp->baseline_map && pp->point_gemm && point_gemm_supported<T>(*d)) {
-        // point sources on the matrix cores into the running-sum buffer,
-        // which the kernel below continues from
+    const bool shaped = shaped_gemm_pays<T>(*d);
+    if (pp && pp->baseline_map && pp->point_gemm && point_gemm_supported<T>(*d) &&
+        (dext(d->npsrc) > 0 || (shaped && dext(d->nsrc) > dext(d->npsrc)))) {
+        // point sources (and where it pays the gaussian and sersic ones)
+        // on the matrix cores into the running-sum buffer; the kernel
+        // below adds the remaining sources and applies gains, flags, chi^2
         cx<T> *vbuf = A.ekb ? A.ekb : A.model;
-        r = launch_point_gemm<T>(A, pp->baseline_map, vbuf, A.vin, stream);
+        r = launch_point_gemm<T>(A, pp->baseline_map, vbuf, A.vin, shaped, stream);
         if (r) return r;
         A.vin = vbuf;
-        A.s_begin = dext(d->npsrc);
+        A.s_begin = shaped ? dext(d->nsrc) : dext(d->npsrc);
     }
     A.ct = p.ct; A.st = p.st; A.nbchunk = p.nbchunk; A.bchunk = p.bchunk; A.nctile = p.nctile;
     if (p.blocked) {

@@ -32,40 +32,22 @@
 
 namespace mbx {
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct Mfma;
-template <> struct Mfma<float> {
-    typedef f32x4 acc_t;
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // output element r of a lane: row 4 * (lane / 16) + r, column lane % 16
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-template <> struct Mfma<double> {
-    typedef f64x4 acc_t;
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // output element r of a lane: row 4 * r + lane / 16, column lane % 16
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
-};
-
 // One row of X for one source: Re J_i0, Re J_i1, Im J_i0, Im J_i1.
 template <typename T> struct alignas(4 * sizeof(T)) XRow { T v[4]; };
 
-// Four consumer waves (one per SIMD, each with its matrix core) and the
-// producer waves the register file leaves room for: 12 in float (1024
-// threads, 128 VGPRs each), 4 in float64 (512 threads, whose consumers
-// hold 144 VGPRs of accumulators). The Jones producers are bound by the
-// latency of the beam gathers, so more of them overlap more.
-template <typename T> constexpr int gemm_threads() { return sizeof(T) == 4 ? 1024 : 512; }
-constexpr int kGemmConsumers = 256;
-constexpr int kGemmMaxTp = 9;         // tile pairs per consumer wave
-constexpr int kGemmMaxRowBlocks = 8;  // 4 * 9 >= 8 * 9 / 2: na <= 64
-constexpr size_t kGemmLdsMax = 160 * 1024;
+// Wave layout per pass. Point pass: four consumer waves (one per SIMD and
+// its matrix core) own up to nine tile pairs each; producers fill the
+// workgroup -- 12 in float (1024 threads, 128 VGPRs each), 4 in float64
+// (512 threads, 256 VGPRs: the consumers hold 144 of accumulators).
+// Shaped pass, whose consumers also evaluate a shape factor per element:
+// eight consumer waves with up to five tile pairs, 8 producers in float
+// (1024 threads), 4 in float64 (768 threads, 168 VGPRs).
+template <typename T, bool SHAPED> constexpr int gemm_threads() {
+    return sizeof(T) == 4 ? 1024 : (SHAPED ? 768 : 512);
+}
+template <bool SHAPED> constexpr int gemm_consumers() { return SHAPED ? 512 : 256; }
+template <bool SHAPED> constexpr int gemm_max_tp() { return SHAPED ? 5 : 9; }
+constexpr int kGemmMaxRowBlocks = 8;  // 4 * 9 and 8 * 5 >= 8 * 9 / 2: na <= 64
 // beam taps a producer keeps in flight
 template <typename T> constexpr int kGemmTaps() { return sizeof(T) == 4 ? 4 : 2; }
 
@@ -80,7 +62,8 @@ static __host__ __device__ inline size_t gemm_x_offset(int na) {
 
 template <typename T>
 static inline size_t gemm_lds_bytes(int na, int st) {
-    return gemm_x_offset(na) + 2 * size_t(st) * gemm_rows(na) * sizeof(XRow<T>);
+    return gemm_x_offset(na) + 2 * size_t(st) * gemm_rows(na) * sizeof(XRow<T>) +
+           2 * 2 * size_t(na) * st * sizeof(T) + 2 * size_t(st) * sizeof(int);
 }
 
 template <typename T>
@@ -90,20 +73,33 @@ static inline int gemm_st(int na, size_t budget) {
     return st;
 }
 
-// LDS a workgroup may take on the current device (160 KB on gfx950).
-static size_t gemm_lds_budget() {
-    int dev = 0, bytes = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-        bytes <= 0)
-        return 64 * 1024;
-    return std::min<size_t>(size_t(bytes), kGemmLdsMax);
+// Shape factor of one (p, q) element from the per-antenna shape
+// coordinates (x, y): gaussian exp(-(dx^2 + dy^2)), the coordinates
+// pre-scaled by sqrt(log2 e) in float so that exp2 applies; sersic
+// (1 + dx^2 + dy^2)^(-3/2).
+template <typename T> __device__ __forceinline__ T shape_f(bool gauss, T dx, T dy);
+template <> __device__ __forceinline__ float shape_f<float>(bool gauss, float dx, float dy) {
+    const float r2 = fmaf(dx, dx, dy * dy);
+    if (gauss) return __builtin_amdgcn_exp2f(-r2);
+    const float rs = __builtin_amdgcn_rsqf(1.0f + r2);
+    return rs * rs * rs;
+}
+template <> __device__ __forceinline__ double shape_f<double>(bool gauss, double dx, double dy) {
+    const double r2 = fma(dx, dx, dy * dy);
+    if (gauss) return exp(-r2);
+    const double den = 1.0 + r2;
+    return 1.0 / (den * sqrt(den));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(gemm_threads<T>())
+// SHAPED = false: the point sources [0, npsrc); true: the gaussian and
+// sersic sources [npsrc, nsrc). Separate instances keep each one's
+// registers to what it needs.
+template <typename T, bool SHAPED>
+__global__ void __launch_bounds__((gemm_threads<T, SHAPED>()))
 k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
-    constexpr int kGemmThreads = gemm_threads<T>();
+    constexpr int kGemmThreads = gemm_threads<T, SHAPED>();
+    constexpr int kGemmConsumers = gemm_consumers<SHAPED>();
+    constexpr int kGemmMaxTp = gemm_max_tp<SHAPED>();
     extern __shared__ __align__(32) unsigned char smem_raw[];
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
@@ -111,8 +107,7 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     const int tid = threadIdx.x;
     // Time fastest: the dispatcher deals consecutive workgroups to the
     // eight XCDs in turn, so the workgroups one XCD runs together share a
-    // channel and with it the two beam-cube frequency planes they gather
-    // from (L2 per XCD).
+    // channel and with it the beam plane they gather from (L2 per XCD).
     const uint64_t w = blockIdx.x;
     if (w >= uint64_t(nt_ext) * nc_ext) return;
     const int t = int(w % nt_ext), c = int(w / nt_ext);
@@ -122,30 +117,29 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     double *Lpa = Luvw + 3 * NA;
     double *Lant = Lpa + 2 * NA;
     double *Lch = Lant + 4 * NA;
-    XRow<T> *X = reinterpret_cast<XRow<T> *>(smem_raw + gemm_x_offset(NA));
-    const int xstride = st * rows;   // one buffer
+    // two source-tile buffers: Jones (point tiles X[s][row], shaped tiles
+    // component-major Xg[k][row][s]), shape coordinates XY[2][a][s] and
+    // the source kinds of shaped tiles
+    T *Xbase = reinterpret_cast<T *>(smem_raw + gemm_x_offset(NA));
+    const int xstride = 4 * st * rows;       // T per Jones buffer
+    T *XYbase = Xbase + 2 * xstride;
+    const int xystride = 2 * NA * st;        // T per coordinate buffer
+    int *Kbase = reinterpret_cast<int *>(XYbase + 2 * xystride);
 
     tile_setup<T>(A, Luvw, Lpa, Lch, t, c, tid, kGemmThreads);
-    {
-        const uint64_t NC = d.nchan.local_size;
-        const int cc = min(c, nc_ext - 1);
-        for (int a = tid; a < NA; a += kGemmThreads) {
-            const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + cc) * 2;
-            const T *asc = A.ascale + (uint64_t(a) * NC + cc) * 2;
-            Lant[4 * a] = double(pet[0]); Lant[4 * a + 1] = double(pet[1]);
-            Lant[4 * a + 2] = double(asc[0]); Lant[4 * a + 3] = double(asc[1]);
-        }
-    }
-    // rows past 2 na stay zero in both buffers
-    for (int i = tid; i < 2 * st * (rows - 2 * NA); i += kGemmThreads) {
-        const int per = rows - 2 * NA;
-        const int bs = i / per;           // buffer * st + source
-        X[bs * rows + 2 * NA + i % per] = XRow<T>{{T(0), T(0), T(0), T(0)}};
-    }
+    stage_antennas<T>(A, Lant, t, c, tid, kGemmThreads);
+    // Jones rows past 2 na read as zero: clear both buffers once (point
+    // and shaped tiles write rows < 2 na only)
+    for (int i = tid; i < 2 * xstride; i += kGemmThreads) Xbase[i] = T(0);
     __syncthreads();
 
-    const int np = dext(d.npsrc);
-    const int ntiles = (np + st - 1) / st;
+    const int np = dext(d.npsrc), ns = dext(d.nsrc);
+    const int sbeg = SHAPED ? np : 0, send = SHAPED ? ns : np;
+    const int ntiles = (send - sbeg + st - 1) / st;
+    auto tile_range = [&](int it, int &s0, int &nst) {
+        s0 = sbeg + it * st;
+        nst = min(st, send - s0);
+    };
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (wave < kGemmConsumers / 64) {
         typedef typename Mfma<T>::acc_t acc_t;
@@ -172,22 +166,72 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
             are[j] = acc_t{T(0), T(0), T(0), T(0)};
             aim[j] = acc_t{T(0), T(0), T(0), T(0)};
         }
+        // Im part of a shaped tile's product: lane group g supplies
+        // component (g + 2) mod 4 of A' = [Im X | -Re X]
+        const acc_t zero = acc_t{T(0), T(0), T(0), T(0)};
         __syncthreads();                      // tile 0 produced
         for (int it = 0; it < ntiles; ++it) {
-            const XRow<T> *Xb = X + (it & 1) * xstride;
-            const int nst = min(st, np - it * st);
-            for (int s0 = 0; s0 < nst; s0 += 4) {
-                const XRow<T> *Xs = Xb + (s0 + g) * rows + i16;
+            const int buf = it & 1;
+            int s0t, nst;
+            tile_range(it, s0t, nst);
+            if (!SHAPED) {
+                const XRow<T> *Xb = reinterpret_cast<const XRow<T> *>(Xbase + buf * xstride);
+                for (int s0 = 0; s0 < nst; s0 += 4) {
+                    const XRow<T> *Xs = Xb + (s0 + g) * rows;
 #pragma unroll
-                for (int j = 0; j < kGemmMaxTp; ++j) {
-                    if (j < ntp_mine) {
-                        const XRow<T> a = Xs[16 * tpR[j]];
-                        const XRow<T> b = Xs[16 * tpC[j]];
-                        const T ai[4] = {a.v[2], a.v[3], -a.v[0], -a.v[1]};
+                    for (int j = 0; j < kGemmMaxTp; ++j) {
+                        if (j < ntp_mine) {
+                            const XRow<T> a = Xs[16 * tpR[j] + Mfma<T>::arow(i16)];
+                            const XRow<T> b = Xs[16 * tpC[j] + i16];
+                            const T ai[4] = {a.v[2], a.v[3], -a.v[0], -a.v[1]};
 #pragma unroll
-                        for (int m = 0; m < 4; ++m) {
-                            are[j] = Mfma<T>::mma(a.v[m], b.v[m], are[j]);
-                            aim[j] = Mfma<T>::mma(ai[m], b.v[m], aim[j]);
+                            for (int m = 0; m < 4; ++m) {
+                                are[j] = Mfma<T>::mma(a.v[m], b.v[m], are[j]);
+                                aim[j] = Mfma<T>::mma(ai[m], b.v[m], aim[j]);
+                            }
+                            // float64 operands take twice the registers: one
+                            // tile pair's loads in flight at a time
+                            if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                }
+            } else {
+                // one source per MFMA (k = the four reals of its Jones
+                // row), each product scaled per element by f_pq
+                const T *Xg = Xbase + buf * xstride;
+                const T *Xc = XYbase + buf * xystride, *Yc = Xc + NA * st;
+                const int *kind = Kbase + buf * st;
+                for (int s = 0; s < nst; ++s) {
+                    // re-derived per source so the per-tile-pair addresses
+                    // are not all hoisted out of the loop into registers
+                    int lv = lane;
+                    __asm__ volatile("" : "+v"(lv));
+                    const int i16v = lv & 15, gv = lv >> 4;
+                    const int g2 = (gv + 2) & 3;
+                    const T sg = gv < 2 ? T(1) : T(-1);
+                    const bool gauss = kind[s] == 1;
+#pragma unroll
+                    for (int j = 0; j < kGemmMaxTp; ++j) {
+                        if (j < ntp_mine) {
+                            const int ra = 16 * tpR[j] + Mfma<T>::arow(i16v), cb = 16 * tpC[j] + i16v;
+                            const acc_t P = Mfma<T>::mma(Xg[(gv * rows + ra) * st + s],
+                                                         Xg[(gv * rows + cb) * st + s], zero);
+                            const acc_t Q = Mfma<T>::mma(sg * Xg[(g2 * rows + ra) * st + s],
+                                                         Xg[(gv * rows + cb) * st + s], zero);
+                            const int q = min(cb >> 1, NA - 1);
+                            const T xq = Xc[q * st + s], yq = Yc[q * st + s];
+                            // this lane's rows are antennas p0 = 8 R + 2 g and p0 + 1
+                            // (rows past 2 na: any finite factor, discarded)
+                            const int p0 = 8 * tpR[j] + 2 * gv;
+                            const int pa0 = min(p0, NA - 1), pa1 = min(p0 + 1, NA - 1);
+                            const T f0 = shape_f<T>(gauss, xq - Xc[pa0 * st + s], yq - Yc[pa0 * st + s]);
+                            const T f1 = shape_f<T>(gauss, xq - Xc[pa1 * st + s], yq - Yc[pa1 * st + s]);
+                            are[j][0] = fma(f0, P[0], are[j][0]); aim[j][0] = fma(f0, Q[0], aim[j][0]);
+                            are[j][1] = fma(f0, P[1], are[j][1]); aim[j][1] = fma(f0, Q[1], aim[j][1]);
+                            are[j][2] = fma(f1, P[2], are[j][2]); aim[j][2] = fma(f1, Q[2], aim[j][2]);
+                            are[j][3] = fma(f1, P[3], are[j][3]); aim[j][3] = fma(f1, Q[3], aim[j][3]);
+                            // one tile pair at a time: bounds the live registers
+                            __builtin_amdgcn_sched_barrier(0);
                         }
                     }
                 }
@@ -203,7 +247,7 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
             const int q = col >> 1, kq = col & 1;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = 16 * tpR[j] + Mfma<T>::row(lane, r);
+                const int row = 16 * tpR[j] + out_row(lane, r);
                 const int p = row >> 1, ip = row & 1;
                 if (p >= NA || q >= NA || p > q) continue;
                 const int e = G.bmap[p * NA + q];
@@ -220,48 +264,92 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
         }
     } else {
         const int ptid = tid - kGemmConsumers, pth = kGemmThreads - kGemmConsumers;
-        const uint64_t NT = d.ntime.local_size;
-        // A producer thread owns one source of the tile and a stride of
-        // antennas: the source's parameters and sqrt(B) are loaded once,
-        // and each Jones then needs only the beam taps from memory.
-        auto produce = [&](int buf, int s0, int nst) {
-            XRow<T> *Xb = X + buf * xstride;
-            const int astep = pth / nst;
-            const int si = ptid % nst, a0 = ptid / nst;
-            if (a0 < astep) {
-                const int s = s0 + si;
-                const double l0 = A.lm[2 * uint64_t(s)], m0 = A.lm[2 * uint64_t(s) + 1];
-                const double nm1 = sqrt(1.0 - l0 * l0 - m0 * m0) - 1.0;
-                const T *S = A.stokes + (uint64_t(s) * NT + t) * 4;
-                const T power = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * Lch[4]);
-                const J4<T> B = brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
-                const double kf = -A.two_pi_over_c * Lch[3];
-                for (int a = a0; a < NA; a += astep) {
-                    const double *an = Lant + 4 * a;
-                    const J4<T> E = beam_jones_at<T, kGemmTaps<T>()>(A, Lpa[2 * a], Lpa[2 * a + 1], an[0],
-                                                                    an[1], an[2], an[3], c, l0, m0);
-                    const double *x = Luvw + 3 * a;
-                    T sn, cs;
-                    PhaseCS<T>::eval(kf * (x[2] * nm1 + x[1] * m0 + x[0] * l0), &sn, &cs);
-                    const cx<T> K = {cs, sn};
-                    J4<T> KB;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
-                    const J4<T> J = jmul(E, KB);
-                    Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
-                    Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
-                }
-            }
-            // sources past the tile's end up to a multiple of four read as zero
-            const int pad = ((nst + 3) & ~3) - nst;
-            for (int i = ptid; i < pad * rows; i += pth)
-                Xb[(nst + i / rows) * rows + i % rows] = XRow<T>{{T(0), T(0), T(0), T(0)}};
+        const uint64_t NGS = d.ngsrc.local_size, NSS = d.nssrc.local_size;
+        const int ng = dext(d.ngsrc);
+        auto jones = [&](int a, double l0, double m0, double nm1, const J4<T> &B) {
+            return staged_jones<T, kGemmTaps<T>()>(A, Luvw, Lpa, Lant, Lch, c, a, l0, m0, nm1, B);
         };
-        if (ntiles > 0) produce(0, 0, min(st, np));
+        auto source_b = [&](int s, double &l0, double &m0, double &nm1) {
+            return source_sqrtb<T>(A, Lch, t, s, l0, m0, nm1);
+        };
+        auto produce = [&](int it, int buf) {
+            int s0, nst;
+            tile_range(it, s0, nst);
+            const int nst4 = (nst + 3) & ~3;
+            if (!SHAPED) {
+                // X[s][row]; a group of consecutive threads shares a source
+                // and walks the antennas (conflict-free 16/32-byte writes)
+                XRow<T> *Xb = reinterpret_cast<XRow<T> *>(Xbase + buf * xstride);
+                const int gs = pth / nst;
+                const int si = ptid / gs, a0 = ptid % gs;
+                if (si < nst) {
+                    double l0, m0, nm1;
+                    const J4<T> B = source_b(s0 + si, l0, m0, nm1);
+                    for (int a = a0; a < NA; a += gs) {
+                        const J4<T> J = jones(a, l0, m0, nm1, B);
+                        Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
+                        Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
+                    }
+                }
+                // sources past the tile's end up to a multiple of four read as zero
+                for (int i = ptid; i < (nst4 - nst) * rows; i += pth)
+                    Xb[(nst + i / rows) * rows + i % rows] = XRow<T>{{T(0), T(0), T(0), T(0)}};
+            } else {
+                // Xg[k][row][s], XY[2][a][s]: consecutive threads take
+                // consecutive sources (conflict-free scalar writes)
+                T *Xg = Xbase + buf * xstride;
+                T *Xc = XYbase + buf * xystride, *Yc = Xc + NA * st;
+                int *kind = Kbase + buf * st;
+                const int astep = pth / nst;
+                const int si = ptid % nst, a0 = ptid / nst;
+                if (a0 < astep) {
+                    const int s = s0 + si;
+                    double l0, m0, nm1;
+                    const J4<T> B = source_b(s, l0, m0, nm1);
+                    // shape coordinates: linear in the antenna's (u, v)
+                    double cu, cv, du, dv;        // x = cu u + cv v, y = du u + dv v
+                    const bool gauss = s < np + ng;
+                    if (gauss) {
+                        const int gi = s - np;
+                        const double el = A.gauss[gi], em = A.gauss[gi + NGS], R = A.gauss[gi + 2 * NGS];
+                        const double k = A.gauss_scale * Lch[3] * (sizeof(T) == 4 ? 1.2011224087864498 : 1.0);
+                        cu = k * R * em; cv = -k * R * el; du = k * el; dv = k * em;
+                    } else {
+                        const int si2 = s - np - ng;
+                        const double e1 = A.sersic[si2], e2 = A.sersic[si2 + NSS];
+                        const double Rp = A.sersic[si2 + 2 * NSS] / (1.0 - e1 * e1 - e2 * e2);
+                        const double k = A.two_pi_over_c * Lch[3] * Rp;
+                        cu = k * (1.0 + e1); cv = k * e2; du = k * e2; dv = k * (1.0 - e1);
+                    }
+                    if (a0 == 0) kind[si] = gauss ? 1 : 2;
+                    for (int a = a0; a < NA; a += astep) {
+                        const J4<T> J = jones(a, l0, m0, nm1, B);
+                        const T jr[2][4] = {{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im},
+                                            {J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
+#pragma unroll
+                        for (int i = 0; i < 2; ++i)
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) Xg[(k * rows + 2 * a + i) * st + si] = jr[i][k];
+                        const double u = Luvw[3 * a], v = Luvw[3 * a + 1];
+                        Xc[a * st + si] = T(cu * u + cv * v);
+                        Yc[a * st + si] = T(du * u + dv * v);
+                    }
+                }
+                // padding sources: zero Jones (no contribution) and zero
+                // coordinates (a finite factor)
+                const int pad = nst4 - nst;
+                for (int i = ptid; i < pad * (4 * rows + 2 * NA); i += pth) {
+                    const int sp = nst + i % pad, r = i / pad;
+                    if (r < 4 * rows) Xg[r * st + sp] = T(0);
+                    else Xc[(r - 4 * rows) * st + sp] = T(0);   // Xc, then Yc rows
+                }
+                for (int i = ptid; i < pad; i += pth) kind[nst + i] = 1;
+            }
+        };
+        if (ntiles > 0) produce(0, 0);
         __syncthreads();
         for (int it = 0; it < ntiles; ++it) {
-            const int s1 = (it + 1) * st;
-            if (it + 1 < ntiles) produce((it + 1) & 1, s1, min(st, np - s1));
+            if (it + 1 < ntiles) produce(it + 1, (it + 1) & 1);
             __syncthreads();
         }
     }
@@ -270,12 +358,21 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
 template <typename T>
 bool point_gemm_supported(const mbx_dims &d) {
     const int na = d.na.local_size;
-    return dext(d.npsrc) > 0 && na > 0 && gemm_rows(na) / 16 <= kGemmMaxRowBlocks;
+    return dext(d.nsrc) > 0 && na > 0 && gemm_rows(na) / 16 <= kGemmMaxRowBlocks;
+}
+
+template <typename T>
+bool shaped_gemm_pays(const mbx_dims &d) {
+    // Measured on MI355X: the per-element shape factors keep the shaped
+    // pass VALU-bound; it beats the vector-unit kernel only in float64
+    // with many antennas (64 antennas: 52 ms against 69 ms for 600
+    // gaussian + sersic sources; 27 antennas, and float32, are slower).
+    return sizeof(T) == 8 && d.na.local_size >= 48;
 }
 
 template <typename T>
 int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, const cx<T> *vin,
-                      hipStream_t stream) {
+                      bool shaped, hipStream_t stream) {
     const mbx_dims &d = A.d;
     const int na = d.na.local_size;
     GemmArgs<T> G;
@@ -283,24 +380,41 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
     G.rows = gemm_rows(na);
     const int nR = G.rows / 16;
     G.ntp = nR * (nR + 1) / 2;
-    G.tpw = (G.ntp + kGemmConsumers / 64 - 1) / (kGemmConsumers / 64);
-    if (G.tpw > kGemmMaxTp) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
     static size_t budget = gemm_lds_budget();
     G.st = gemm_st<T>(na, budget);
     const uint64_t nwg = uint64_t(dext(d.ntime)) * dext(d.nchan);
     RimeArgs<T> Ag = A;
     Ag.ct = 1;
     const size_t lds = gemm_lds_bytes<T>(na, G.st);
-    hipLaunchKernelGGL(k_point_gemm<T>, dim3(unsigned(nwg)), dim3(gemm_threads<T>()), lds, stream,
-                       Ag, G);
-    return hip_status(hipGetLastError(), "point gemm launch");
+    const int np = dext(d.npsrc), ns = dext(d.nsrc);
+    if (np > 0) {
+        constexpr int cw = gemm_consumers<false>() / 64;
+        G.tpw = (G.ntp + cw - 1) / cw;
+        if (G.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
+        hipLaunchKernelGGL((k_point_gemm<T, false>), dim3(unsigned(nwg)), dim3(gemm_threads<T, false>()),
+                           lds, stream, Ag, G);
+        int r = hip_status(hipGetLastError(), "point gemm launch");
+        if (r) return r;
+        G.vin = dst;   // the shaped pass continues from the point sums
+    }
+    if (shaped && ns > np) {
+        constexpr int cw = gemm_consumers<true>() / 64;
+        G.tpw = (G.ntp + cw - 1) / cw;
+        if (G.tpw > gemm_max_tp<true>()) return set_error(MBX_E_UNSUPPORTED, "shaped gemm: %d antennas", na);
+        hipLaunchKernelGGL((k_point_gemm<T, true>), dim3(unsigned(nwg)), dim3(gemm_threads<T, true>()),
+                           lds, stream, Ag, G);
+        return hip_status(hipGetLastError(), "shaped gemm launch");
+    }
+    return MBX_OK;
 }
 
 template bool point_gemm_supported<float>(const mbx_dims &);
 template bool point_gemm_supported<double>(const mbx_dims &);
+template bool shaped_gemm_pays<float>(const mbx_dims &);
+template bool shaped_gemm_pays<double>(const mbx_dims &);
 template int launch_point_gemm<float>(const RimeArgs<float> &, const int32_t *, cx<float> *,
-                                      const cx<float> *, hipStream_t);
+                                      const cx<float> *, bool, hipStream_t);
 template int launch_point_gemm<double>(const RimeArgs<double> &, const int32_t *, cx<double> *,
-                                       const cx<double> *, hipStream_t);
+                                       const cx<double> *, bool, hipStream_t);
 
 }  // namespace mbx

@@ -6,6 +6,94 @@
 
 namespace mbx {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_*_16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
+// Output element r of lane l is, in float, D[4 (l / 16) + r][l % 16] and,
+// in float64, D[4 r + l / 16][l % 16]. For float64 the A rows are loaded
+// permuted (lane i of a group supplies the tile's row arow(i)) so that
+// both types end with the same layout: element r of lane l is tile row
+// 4 (l / 16) + r, i.e. two antennas (rows 2p, 2p + 1) per lane.
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+    typedef f32x4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int arow(int i) { return i; }
+};
+template <> struct Mfma<double> {
+    typedef f64x4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
+};
+// tile row of output element r of a lane (both types, see above)
+__device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+
+// LDS a workgroup may take on the current device (160 KB on gfx950).
+constexpr size_t kGemmLdsMax = 160 * 1024;
+inline size_t gemm_lds_budget() {
+    int dev = 0, bytes = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        bytes <= 0)
+        return 64 * 1024;
+    return bytes < int(kGemmLdsMax) ? size_t(bytes) : kGemmLdsMax;
+}
+
+// Source parameters (l, m, n - 1) and sqrt(B) (f/f_ref)^alpha of source s
+// at time t for the workgroup's channel record Lch.
+template <typename T>
+__device__ __forceinline__ J4<T> source_sqrtb(const RimeArgs<T> &A, const double *Lch, int t, int s,
+                                             double &l0, double &m0, double &nm1) {
+    const uint64_t NT = A.d.ntime.local_size;
+    l0 = A.lm[2 * uint64_t(s)]; m0 = A.lm[2 * uint64_t(s) + 1];
+    nm1 = sqrt(1.0 - l0 * l0 - m0 * m0) - 1.0;
+    const T *S = A.stokes + (uint64_t(s) * NT + t) * 4;
+    const T power = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * Lch[4]);
+    return brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
+}
+
+// J = E K sqrt(B) of one (source, antenna a) for the workgroup's channel c
+// from LDS-staged per-antenna records: uvw [3], parallactic sin/cos [2],
+// pointing error and scaling [4], channel record Lch.
+template <typename T, int UT>
+__device__ __forceinline__ J4<T> staged_jones(const RimeArgs<T> &A, const double *Luvw,
+                                              const double *Lpa, const double *Lant,
+                                              const double *Lch, int c, int a, double l0,
+                                              double m0, double nm1, const J4<T> &B) {
+    const double *an = Lant + 4 * a;
+    const J4<T> E = beam_jones_at<T, UT>(A, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2], an[3],
+                                         c, l0, m0);
+    const double *x = Luvw + 3 * a;
+    T sn, cs;
+    PhaseCS<T>::eval(-A.two_pi_over_c * Lch[3] * (x[2] * nm1 + x[1] * m0 + x[0] * l0), &sn, &cs);
+    const cx<T> K = {cs, sn};
+    J4<T> KB;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
+    return jmul(E, KB);
+}
+
+// Per-antenna pointing error and scaling of time t, channel c into LDS.
+template <typename T>
+__device__ __forceinline__ void stage_antennas(const RimeArgs<T> &A, double *Lant, int t, int c,
+                                               int tid, int nth) {
+    const mbx_dims &d = A.d;
+    const int NA = d.na.local_size;
+    const uint64_t NC = d.nchan.local_size;
+    for (int a = tid; a < NA; a += nth) {
+        const T *pet = A.pe + ((uint64_t(t) * NA + a) * NC + c) * 2;
+        const T *asc = A.ascale + (uint64_t(a) * NC + c) * 2;
+        Lant[4 * a] = double(pet[0]); Lant[4 * a + 1] = double(pet[1]);
+        Lant[4 * a + 2] = double(asc[0]); Lant[4 * a + 3] = double(asc[1]);
+    }
+}
+
 template <typename T> struct GemmArgs {
     const int32_t *bmap;   // (na x na) pair -> baseline map, device
     cx<T> *dst;            // running source sum, (ntime, nbl, nchan, 4)
@@ -16,10 +104,14 @@ template <typename T> struct GemmArgs {
 // True when the antenna count fits the kernel's tile assignment.
 template <typename T> bool point_gemm_supported(const mbx_dims &d);
 
+// Whether the gaussian and sersic sources are also worth the matrix cores.
+template <typename T> bool shaped_gemm_pays(const mbx_dims &d);
+
 // dst[t, b, c] = (vin ? vin[t, b, c] : 0) + sum over the point sources
-// [0, npsrc extent) of J_p J_q^H, for every baseline of the map.
+// [0, npsrc extent) of J_p J_q^H -- and with `shaped` over the gaussian
+// and sersic sources too, f_pq J_p J_q^H -- for every baseline of the map.
 template <typename T>
 int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, const cx<T> *vin,
-                      hipStream_t stream);
+                      bool shaped, hipStream_t stream);
 
 }  // namespace mbx

@@ -96,8 +96,12 @@ def test_batched_launch_shapes_agree(tune):
 
 
 def test_batched_64_antennas_matches_single_model_path():
-    """At 64 antennas (2016 baselines) the batched float32 results equal the
-    single-model forward and adjoint run per sample."""
+    """At 64 antennas (2016 baselines) the batched float32 results and the
+    single-model float32 forward + adjoint run per sample both agree with
+    the float64 reference gradient to the float32 bound. The gradient
+    carries the residual R = O - M, whose float32 error is that of the
+    model, so the bound scales with kappa = |M|max / |R|max:
+    max(1e-4, 3e-6 kappa) relative (the smaller tests have kappa < 30)."""
     from montblanc_amd import batched, rime
     counts = [(40, 10, 0), (5, 2, 0), (200, 60, 0)]
     shared, samples = make_batch(na=64, ntime=2, nchan=3, counts=counts, seed=33)
@@ -105,14 +109,22 @@ def test_batched_64_antennas_matches_single_model_path():
     X2, g = batched.evaluate(b, gradients=True, tune=None)
     torch.cuda.synchronize()
     for k, s in enumerate(samples):
-        arrays = rime.to_device(synth.as_dtype(sample_inputs(shared, s), np.float32))
+        a32 = synth.as_dtype(sample_inputs(shared, s), np.float32)
+        ref = rime_torch.gradients(synth.upcast(a32), *s['counts'], sigma_sqrd=shared['sigma_sqrd'])
+        arrays = rime.to_device(a32)
         out = rime.forward(arrays, s['counts'], keep_ekb=True)
         gg = rime.adjoint(arrays, s['counts'], out['ekb_vis'])
         torch.cuda.synchronize()
         assert abs(X2[k].item() - out['X2'].item()) < 1e-6 * out['X2'].item()
         o0, o1 = b.src_offset[k], b.src_offset[k + 1]
+        up = synth.upcast(a32)
+        M = rime_np.rime(up, *s['counts'])['model_vis']
+        R = (up['observed_vis'] - M) * (up['flag'] == 0)
+        tol = max(1e-4, 3e-6 * np.abs(M).max() / np.abs(R).max())
+        print('sample', k, 'kappa', np.abs(M).max() / np.abs(R).max(), flush=True)
         for name in ('lm', 'stokes', 'alpha'):
-            assert rel(g[name][o0:o1].cpu(), gg[name].cpu()) < 1e-4, (k, name)
+            assert rel(g[name][o0:o1].cpu().numpy(), ref[name]) < tol, (k, name, 'batched')
+            assert rel(gg[name].cpu().numpy(), ref[name]) < tol, (k, name, 'single')
 
 
 def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
