@@ -75,8 +75,20 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     double *Lch = Lant + 4 * NA;
     T *Wr = reinterpret_cast<T *>(smem_raw + adjg_setup_bytes(NA));
     T *Wi = Wr + rows * rows;
-    T *Xr = Wi + rows * rows;                 // [2 st][rows]: X^T, n = 2 s + j
+    // X (re, im planes): point pass [n][row] (MFMA operand reads along the
+    // rows), shaped pass [row][n] (a wave reads one row for many sources);
+    // n = 2 s + j, row = 2 a + k
+    T *Xr = Wi + rows * rows;
     T *Xi = Xr + 2 * st * rows;
+    // Point pass operands are blocked by four along the summed index, so a
+    // lane's four consecutive k values are one 16/32-byte read and
+    // consecutive lanes (rows of W, sources of X) read consecutive blocks:
+    //   W[row][col] at ((col / 4) rows + row) 4 + col % 4,
+    //   X[n][row]   at ((row / 4) 2 st + n) 4 + row % 4   (point pass)
+    auto xi_ = [&](int n, int row) {
+        return SHAPED ? row * (2 * st) + n : ((row >> 2) * (2 * st) + n) * 4 + (row & 3);
+    };
+    auto wi_ = [&](int row, int col) { return ((col >> 2) * rows + row) * 4 + (col & 3); };
     J4<T> *Jb = reinterpret_cast<J4<T> *>(Xr); // [st][NA], written over X
     double *sacc = reinterpret_cast<double *>(Xi + 2 * st * rows);   // [st][10]
     double *Lsd = sacc + S.sw * 10;                                 // [st][4]: l, m, n - 1
@@ -100,13 +112,13 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 const cx<T> v = V.v[2 * i + k];
                 if (p == q) {
                     const cx<T> u = V.v[2 * k + i];   // + (Vb^H)[i][k] = conj(Vb[k][i])
-                    Wr[(2 * p + i) * rows + 2 * p + k] = v.re + u.re;
-                    Wi[(2 * p + i) * rows + 2 * p + k] = v.im - u.im;
+                    Wr[wi_(2 * p + i, 2 * p + k)] = v.re + u.re;
+                    Wi[wi_(2 * p + i, 2 * p + k)] = v.im - u.im;
                 } else {
-                    Wr[(2 * p + i) * rows + 2 * q + k] = v.re;
-                    Wi[(2 * p + i) * rows + 2 * q + k] = v.im;
-                    Wr[(2 * q + k) * rows + 2 * p + i] = v.re;
-                    Wi[(2 * q + k) * rows + 2 * p + i] = -v.im;
+                    Wr[wi_(2 * p + i, 2 * q + k)] = v.re;
+                    Wi[wi_(2 * p + i, 2 * q + k)] = v.im;
+                    Wr[wi_(2 * q + k, 2 * p + i)] = v.re;
+                    Wi[wi_(2 * q + k, 2 * p + i)] = -v.im;
                 }
             }
     }
@@ -138,7 +150,9 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         __syncthreads();
         // ---- X of the tile (columns of sources past nst are zero)
         for (int i = tid; i < nst * NA; i += kAdjThreads) {
-            const int si = i % nst, a = i / nst;
+            // point pass: consecutive threads take consecutive antennas
+            // (conflict-free [n][row] writes); shaped: consecutive sources
+            const int si = SHAPED ? i % nst : i / NA, a = SHAPED ? i / nst : i % NA;
             const J4<T> B = reinterpret_cast<const J4<T> *>(Lst + 16 * si + 8)[0];
             const J4<T> J = staged_jones<T, 2>(R, Luvw, Lpa, Lant, Lch, c, a, Lsd[4 * si],
                                                Lsd[4 * si + 1], Lsd[4 * si + 2], B);
@@ -146,12 +160,13 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             for (int k = 0; k < 2; ++k)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    Xr[(2 * si + j) * rows + 2 * a + k] = J.v[2 * k + j].re;
-                    Xi[(2 * si + j) * rows + 2 * a + k] = J.v[2 * k + j].im;
+                    Xr[xi_(2 * si + j, 2 * a + k)] = J.v[2 * k + j].re;
+                    Xi[xi_(2 * si + j, 2 * a + k)] = J.v[2 * k + j].im;
                 }
         }
         for (int i = tid; i < 2 * st * rows; i += kAdjThreads) {
-            const int n = i / rows, r = i % rows;
+            const int n = SHAPED ? i % (2 * st) : (i >> 2) % (2 * st);
+            const int r = SHAPED ? i / (2 * st) : (i / (8 * st)) * 4 + (i & 3);
             if (n >= 2 * nst || r >= 2 * NA) { Xr[i] = T(0); Xi[i] = T(0); }
         }
         if (SHAPED)
@@ -183,16 +198,15 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     const int o = wave + kAdjMfmaWaves * j;
                     if (o >= nout) continue;
                     const int Rb = o / ncb, Cb = o % ncb;
-                    const T *wr = Wr + (16 * Rb + Mfma<T>::arow(i16)) * rows + 4 * g;
-                    const T *wi = Wi + (16 * Rb + Mfma<T>::arow(i16)) * rows + 4 * g;
-                    const T *xr = Xr + (16 * Cb + i16) * rows + 4 * g;
-                    const T *xi = Xi + (16 * Cb + i16) * rows + 4 * g;
+                    const int wrow = 16 * Rb + Mfma<T>::arow(i16), xn = 16 * Cb + i16;
                     for (int kk = 0; kk < rows; kk += 16) {
+                        const T *wr = Wr + wi_(wrow, kk + 4 * g), *wi = Wi + wi_(wrow, kk + 4 * g);
+                        const T *xr = Xr + xi_(xn, kk + 4 * g), *xi = Xi + xi_(xn, kk + 4 * g);
                         T a4[4], b4[4], x4[4], y4[4];
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
-                            a4[m] = wr[kk + m]; b4[m] = wi[kk + m];
-                            x4[m] = xr[kk + m]; y4[m] = xi[kk + m];
+                            a4[m] = wr[m]; b4[m] = wi[m];
+                            x4[m] = xr[m]; y4[m] = xi[m];
                         }
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
@@ -235,7 +249,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 for (int k = 0; k < 2; ++k)
 #pragma unroll
                     for (int j = 0; j < 2; ++j)
-                        Jp.v[2 * k + j] = {Xr[(2 * si + j) * rows + 2 * p + k], Xi[(2 * si + j) * rows + 2 * p + k]};
+                        Jp.v[2 * k + j] = {Xr[xi_(2 * si + j, 2 * p + k)], Xi[xi_(2 * si + j, 2 * p + k)]};
                 const double up = Luvw[3 * p], vp = Luvw[3 * p + 1];
 #pragma unroll 2
                 for (int q = 0; q < NA; ++q) {
@@ -244,8 +258,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int k = 0; k < 2; ++k) {
-                            Wq.v[2 * i + k] = {Wr[(2 * p + i) * rows + 2 * q + k], Wi[(2 * p + i) * rows + 2 * q + k]};
-                            Jq.v[2 * i + k] = {Xr[(2 * si + k) * rows + 2 * q + i], Xi[(2 * si + k) * rows + 2 * q + i]};
+                            Wq.v[2 * i + k] = {Wr[wi_(2 * p + i, 2 * q + k)], Wi[wi_(2 * p + i, 2 * q + k)]};
+                            Jq.v[2 * i + k] = {Xr[xi_(2 * si + k, 2 * q + i)], Xi[xi_(2 * si + k, 2 * q + i)]};
                         }
                     const J4<T> Y = jmul(Wq, Jq);
                     T df[3];
