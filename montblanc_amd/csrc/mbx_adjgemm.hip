@@ -20,10 +20,12 @@
 // sqrt(B) (antenna_backward) and reduce each source's gradient in LDS
 // before one set of float64 atomics per source and workgroup.
 //
-// Per source tile: produce X (all waves) | Jb (MFMA waves, or all waves)
-// | backward (all waves) | flush, with a barrier between the phases.
-// Jb is held in registers across the barrier that ends the product and
-// then written over X's LDS, which the backward no longer needs.
+// Per source tile: source parameters | X (all waves, one (source,
+// antenna) item each, whose forward state -- beam Jones and its l, m
+// derivatives, phase, sqrt(B0) -- stays in registers) | Jb (MFMA waves,
+// or all waves) | backward from the kept state | flush per window, with a
+// barrier between the phases. Jb is held in registers across the barrier
+// that ends the product and then written over X's LDS.
 
 #include "mbx_backward.hpp"
 #include "mbx_gemm.hpp"
@@ -148,20 +150,27 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             reinterpret_cast<J4<T> *>(o + 8)[0] = B;
         }
         __syncthreads();
-        // ---- X of the tile (columns of sources past nst are zero)
-        for (int i = tid; i < nst * NA; i += kAdjThreads) {
-            // point pass: consecutive threads take consecutive antennas
-            // (conflict-free [n][row] writes); shaped: consecutive sources
-            const int si = SHAPED ? i % nst : i / NA, a = SHAPED ? i / nst : i % NA;
-            const J4<T> B = reinterpret_cast<const J4<T> *>(Lst + 16 * si + 8)[0];
-            const J4<T> J = staged_jones<T, 2>(R, Luvw, Lpa, Lant, Lch, c, a, Lsd[4 * si],
-                                               Lsd[4 * si + 1], Lsd[4 * si + 2], B);
+        // ---- X of the tile (columns of sources past nst are zero). Each
+        // thread owns one (source, antenna) item (st * na <= threads) and
+        // keeps its forward state for the backward after the product.
+        // Point pass: consecutive threads take consecutive antennas
+        // (conflict-free [n][row] writes); shaped: consecutive sources.
+        const bool own = tid < nst * NA;
+        const int isi = own ? (SHAPED ? tid % nst : tid / NA) : 0;
+        const int ia = own ? (SHAPED ? tid / nst : tid % NA) : 0;
+        ItemState<T> ist;
+        if (own) {
+            const double *an = Lant + 4 * ia;
+            const T *o = Lst + 16 * isi;
+            const J4<T> J = item_forward<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[0], an[1],
+                                            an[2], an[3], c, Lch[3], Lch[4], Lsd[4 * isi],
+                                            Lsd[4 * isi + 1], o[0], o[1], o[2], o[3], o[4], ist);
 #pragma unroll
             for (int k = 0; k < 2; ++k)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    Xr[xi_(2 * si + j, 2 * a + k)] = J.v[2 * k + j].re;
-                    Xi[xi_(2 * si + j, 2 * a + k)] = J.v[2 * k + j].im;
+                    Xr[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].re;
+                    Xi[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].im;
                 }
         }
         for (int i = tid; i < 2 * st * rows; i += kAdjThreads) {
@@ -251,7 +260,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     for (int j = 0; j < 2; ++j)
                         Jp.v[2 * k + j] = {Xr[xi_(2 * si + j, 2 * p + k)], Xi[xi_(2 * si + j, 2 * p + k)]};
                 const double up = Luvw[3 * p], vp = Luvw[3 * p + 1];
-#pragma unroll 2
+#pragma unroll 1
                 for (int q = 0; q < NA; ++q) {
                     J4<T> Wq, Jq;
 #pragma unroll
@@ -284,16 +293,11 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         }
         __syncthreads();
         // ---- backward of J = E K sqrt(B) per (source, antenna)
-        for (int i = tid; i < nst * NA; i += kAdjThreads) {
-            const int si = i % nst, a = i / nst;
+        if (own) {
             double g[7];
-            const double *an = Lant + 4 * a;
-            const T *o = Lst + 16 * si;
-            antenna_backward_at<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2],
-                                   an[3], c, Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1], o[0], o[1],
-                                   o[2], o[3], o[4], Jb[si * NA + a], g);
+            item_backward<T>(ist, Jb[isi * NA + ia], g);
 #pragma unroll
-            for (int x = 0; x < 7; ++x) atomicAdd(sac + si * 10 + x, g[x]);
+            for (int x = 0; x < 7; ++x) atomicAdd(sac + isi * 10 + x, g[x]);
         }
         __syncthreads();
         if (s0 + st < send && s0 + 2 * st <= wbase + S.sw) continue;   // window not full

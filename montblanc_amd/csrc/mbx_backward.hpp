@@ -53,19 +53,27 @@ __device__ __forceinline__ T jdot_re(const J4<T> &A, const J4<T> &B) {
 // ---------------------------------------------------------------------
 // Per-item backward of J = E K sqrt(B) onto (l, m, I, Q, U, V, alpha).
 
+// What the backward of one (source, antenna, channel) item needs from its
+// forward: the beam Jones and its derivatives along l and m, the phase
+// term, sqrt(B0) of the unscaled brightness and the chain-rule factors.
+template <typename T> struct ItemState {
+    J4<T> E, dEl, dEm, B0s;
+    cx<T> K;
+    T rP, s0, it0, I, Q, U, V;
+    double lnr, dphi_dl, dphi_dm, dvl_dl, dvl_dm, dvm_dl, dvm_dm;
+};
+
+// Forward of one item with the derivative information its backward needs.
 // Inputs: the source's l, m; its Stokes I, Q, U, V and spectral power
 // P = (f/f_ref)^alpha at this (time, channel); the antenna's uvw (xa),
 // parallactic sin/cos (ps, pc), pointing error (pel, pem) and scaling
 // (al, am); the channel c (beam plane), its frequency f and
-// lnr = log(f/f_ref); and Jb. Writes d chi^2 / d(l, m, I, Q, U, V, alpha)
-// of this item to g.
+// lnr = log(f/f_ref). Returns J = E K sqrt(B).
 template <typename T>
-__device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const double *xa,
-                                                    double ps, double pc, double pel, double pem,
-                                                    double al, double am, int c, double f,
-                                                    double lnr, double l0, double m0, T I, T Q,
-                                                    T U, T V, T P, const J4<T> &Jb,
-                                                    double *g /* [7] */) {
+__device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double *xa, double ps,
+                                              double pc, double pel, double pem, double al,
+                                              double am, int c, double f, double lnr, double l0,
+                                              double m0, T I, T Q, T U, T V, T P, ItemState<T> &st) {
     const mbx_dims &d = A.d;
     const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
     const double nm1 = n - 1.0;
@@ -147,6 +155,30 @@ __device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const 
     J4<T> X;
 #pragma unroll
     for (int x = 0; x < 4; ++x) X.v[x] = cmul(K, sB.v[x]);
+    st.E = E; st.dEl = dEl; st.dEm = dEm; st.B0s = B0s; st.K = K;
+    st.rP = rP; st.s0 = s0; st.it0 = it0; st.I = I; st.Q = Q; st.U = U; st.V = V;
+    st.lnr = lnr;
+    st.dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * l0 / n);
+    st.dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * m0 / n);
+    st.dvl_dl = okl ? A.lscale * al * pc : 0.0; st.dvl_dm = okl ? -A.lscale * al * ps : 0.0;
+    st.dvm_dl = okm ? A.mscale * am * ps : 0.0; st.dvm_dm = okm ? A.mscale * am * pc : 0.0;
+    return jmul(E, X);
+}
+
+// Backward of one item: d chi^2 / d(l, m, I, Q, U, V, alpha) into g from
+// the Jones adjoint Jb and the item's forward state.
+template <typename T>
+__device__ __forceinline__ void item_backward(const ItemState<T> &st, const J4<T> &Jb,
+                                              double *g /* [7] */) {
+    const J4<T> &E = st.E, &dEl = st.dEl, &dEm = st.dEm, &B0s = st.B0s;
+    const cx<T> K = st.K;
+    const T rP = st.rP, s0 = st.s0, it0 = st.it0, I = st.I, Q = st.Q, U = st.U, V = st.V;
+    const T t0 = it0 != T(0) ? T(1) : T(0);   // nonzero marker: it0 = 1/t0 or 0
+    const double lnr = st.lnr;
+    const J4<T> sB = jscale(B0s, rP);
+    J4<T> X;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) X.v[x] = cmul(K, sB.v[x]);
     // ---- adjoints
     const J4<T> Eb = jmul_h(Jb, X);                 // Jb X^H
     const J4<T> Xb = jmul(jherm(E), Jb);            // E^H Jb
@@ -182,12 +214,21 @@ __device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const 
         g[2] = g[3] = g[4] = g[5] = 0.0;
     }
     // l, m: phase and beam-weight paths
-    const double dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * l0 / n);
-    const double dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * m0 / n);
-    const double dvl_dl = okl ? A.lscale * al * pc : 0.0, dvl_dm = okl ? -A.lscale * al * ps : 0.0;
-    const double dvm_dl = okm ? A.mscale * am * ps : 0.0, dvm_dm = okm ? A.mscale * am * pc : 0.0;
-    g[0] = double(g_phi) * dphi_dl + double(g_ld) * dvl_dl + double(g_md) * dvm_dl;
-    g[1] = double(g_phi) * dphi_dm + double(g_ld) * dvl_dm + double(g_md) * dvm_dm;
+    g[0] = double(g_phi) * st.dphi_dl + double(g_ld) * st.dvl_dl + double(g_md) * st.dvm_dl;
+    g[1] = double(g_phi) * st.dphi_dm + double(g_ld) * st.dvl_dm + double(g_md) * st.dvm_dm;
+}
+
+// Forward and backward of one item in one call (inputs as item_forward).
+template <typename T>
+__device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const double *xa,
+                                                    double ps, double pc, double pel, double pem,
+                                                    double al, double am, int c, double f,
+                                                    double lnr, double l0, double m0, T I, T Q,
+                                                    T U, T V, T P, const J4<T> &Jb,
+                                                    double *g /* [7] */) {
+    ItemState<T> st;
+    (void)item_forward<T>(A, xa, ps, pc, pel, pem, al, am, c, f, lnr, l0, m0, I, Q, U, V, P, st);
+    item_backward<T>(st, Jb, g);
 }
 
 // The same item reading its inputs from global memory and the tile's
