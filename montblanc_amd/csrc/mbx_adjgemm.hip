@@ -65,10 +65,9 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     const uint64_t NT = d.ntime.local_size, NC = d.nchan.local_size, NBL = d.nbl.local_size;
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan), nb_ext = dext(d.nbl);
     const int tid = threadIdx.x;
-    // time fastest (beam planes shared by the workgroups of one XCD)
-    const uint64_t w = blockIdx.x;
-    if (w >= uint64_t(nt_ext) * nc_ext) return;
-    const int t = int(w % nt_ext), c = int(w / nt_ext);
+    // XCD order of kChanGroup (mbx_tile.hpp)
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
     const int rows = S.rows, st = S.st;
 
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -360,7 +359,7 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
     AdjGemmShape S;
     size_t lds;
     if (!adjg_shape<T>(d, S, lds)) return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: layout");
-    const unsigned nwg = unsigned(uint64_t(dext(d.ntime)) * dext(d.nchan));
+    const unsigned nwg = 8 * tc_per_xcd(dext(d.ntime), dext(d.nchan));
     AdjArgs<T> Ag = A;
     Ag.R.ct = 1;
     if (dext(d.npsrc) > 0) {

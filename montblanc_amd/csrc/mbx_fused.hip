@@ -32,18 +32,21 @@ void launch_chi2_final(const double *partial, unsigned n, double scale, double *
 
 constexpr int kGenThreads = 256;
 
-// Workgroup index -> (time, baseline chunk, channel tile), time fastest:
-// the dispatcher deals consecutive workgroups to the eight XCDs in turn,
-// so the workgroups one XCD runs together share a channel tile and with
-// it the two frequency planes of the beam cube they read (L2 per XCD).
+// Workgroup index -> (time, baseline chunk, channel tile) in the XCD
+// order of kChanGroup (mbx_tile.hpp): channel tile fastest over a group,
+// then time, then baseline chunk, then channel group. False for the
+// padding workgroups past the grid.
 template <typename T>
-__device__ __forceinline__ void wg_coords(const RimeArgs<T> &A, int &bc, int &ctile, int &t) {
-    uint64_t wg = blockIdx.x;
+__device__ __forceinline__ bool wg_coords(const RimeArgs<T> &A, int &bc, int &ctile, int &t) {
+    uint64_t w = xcd_slot(A.per_xcd);
     const uint64_t NT = A.d.ntime.local_size;
-    t = int(wg % NT);
-    wg /= NT;
-    bc = int(wg % A.nbchunk);
-    ctile = int(wg / A.nbchunk);
+    const int ci = int(w % kChanGroup);
+    w /= kChanGroup;
+    t = int(w % NT);
+    w /= NT;
+    bc = int(w % A.nbchunk);
+    ctile = int(w / A.nbchunk) * kChanGroup + ci;
+    return ctile < A.nctile;
 }
 
 // ---------------------------------------------------------------------
@@ -59,7 +62,10 @@ k_forward_gen(RimeArgs<T> A) {
     const int ct = A.ct, st = A.st;
     const int tid = threadIdx.x;
     int bc, ctile, t;
-    wg_coords(A, bc, ctile, t);
+    if (!wg_coords(A, bc, ctile, t)) {       // padding workgroup
+        if (threadIdx.x == 0) A.partial[blockIdx.x] = 0.0;
+        return;
+    }
     Tile<T> L(smem_raw, NA, ct, st);
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan), nb_ext = dext(d.nbl);
     const bool active = t < nt_ext;
@@ -160,7 +166,10 @@ k_forward_blk(RimeArgs<T> A) {
     const int ct = A.ct, st = A.st;
     const int tid = threadIdx.x, nth = blockDim.x;
     int bc, ctile, t;
-    wg_coords(A, bc, ctile, t);
+    if (!wg_coords(A, bc, ctile, t)) {       // padding workgroup
+        if (threadIdx.x == 0) A.partial[blockIdx.x] = 0.0;
+        return;
+    }
     Tile<T> L(smem_raw, NA, ct, st);
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const bool active = t < nt_ext;
@@ -323,7 +332,10 @@ k_forward_pc(RimeArgs<T> A, int cth) {
     const int ct = A.ct, st = A.st;
     const int tid = threadIdx.x, nth = blockDim.x;
     int bc, ctile, t;
-    wg_coords(A, bc, ctile, t);
+    if (!wg_coords(A, bc, ctile, t)) {       // padding workgroup
+        if (threadIdx.x == 0) A.partial[blockIdx.x] = 0.0;
+        return;
+    }
     PcTile<T> L(smem_raw, NA, ct, st);
     const bool active = t < int(dext(d.ntime));
     const int c0 = ctile * ct;
@@ -450,7 +462,8 @@ static FwdPlan plan_forward(const mbx_dims &d, const mbx_pair_plan *pp) {
         while (st > 1 && pc_lds_bytes<T>(na, p.ct, st, p.threads / 64) > kLdsBudget) st >>= 1;
         p.st = st;
         p.lds_bytes = pc_lds_bytes<T>(na, p.ct, st, p.threads / 64);
-        p.nwg = uint64_t(d.ntime.local_size) * p.nctile * p.nbchunk;
+        p.nwg = 8 * uint64_t(xcd_slots(uint64_t(d.ntime.local_size) * p.nbchunk *
+                                        ((p.nctile + kChanGroup - 1) / kChanGroup) * kChanGroup));
         return p;
     }
     if (pp && pp->blocks && pp->nblocks > 0) {
@@ -477,14 +490,17 @@ static FwdPlan plan_forward(const mbx_dims &d, const mbx_pair_plan *pp) {
     p.nctile = (d.nchan.local_size + p.ct - 1) / p.ct;
     p.st = choose_st<T>(na, p.ct, p.threads / 64);
     p.lds_bytes = tile_lds_bytes<T>(na, p.ct, p.st, p.threads / 64);
-    p.nwg = uint64_t(d.ntime.local_size) * p.nctile * p.nbchunk;
+    p.nwg = 8 * uint64_t(xcd_slots(uint64_t(d.ntime.local_size) * p.nbchunk *
+                                    ((p.nctile + kChanGroup - 1) / kChanGroup) * kChanGroup));
     return p;
 }
 
-// Upper bound on workgroups of either kernel: partial buffer size.
+// Upper bound on workgroups of any forward kernel (one float64 partial
+// each): time x (channel tiles padded to kChanGroup) x baseline chunks
+// (at least 64 pair blocks or baselines per chunk), padded to 8 XCDs.
 static uint64_t max_partials(const mbx_dims &d) {
-    uint64_t nb = d.nbl.local_size;
-    return uint64_t(d.ntime.local_size) * d.nchan.local_size * std::max<uint64_t>(1, (nb + 255) / 256);
+    const uint64_t nb = d.nbl.local_size;
+    return uint64_t(d.ntime.local_size) * (d.nchan.local_size + kChanGroup) * ((nb + 63) / 64 + 1) + 8;
 }
 
 template <typename T>
@@ -599,6 +615,10 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
         A.s_begin = shaped ? dext(d->nsrc) : dext(d->npsrc);
     }
     A.ct = p.ct; A.st = p.st; A.nbchunk = p.nbchunk; A.bchunk = p.bchunk; A.nctile = p.nctile;
+    A.per_xcd = uint32_t(p.nwg / 8);
+    if (p.nwg > max_partials(*d))
+        return set_error(MBX_E_UNSUPPORTED, "forward: %llu workgroups exceed the partial buffer",
+                         (unsigned long long)p.nwg);
     if (p.blocked) {
         A.blocks = pp->blocks; A.nblocks = pp->nblocks; A.pb = pp->pb; A.qb = pp->qb;
         launch_blocked<T>(p, pp, A, stream);

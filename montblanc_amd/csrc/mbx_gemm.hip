@@ -105,12 +105,9 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     const int NA = d.na.local_size;
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const int tid = threadIdx.x;
-    // Time fastest: the dispatcher deals consecutive workgroups to the
-    // eight XCDs in turn, so the workgroups one XCD runs together share a
-    // channel and with it the beam plane they gather from (L2 per XCD).
-    const uint64_t w = blockIdx.x;
-    if (w >= uint64_t(nt_ext) * nc_ext) return;
-    const int t = int(w % nt_ext), c = int(w / nt_ext);
+    // XCD order of kChanGroup (mbx_tile.hpp)
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
 
     const int rows = G.rows, st = G.st;
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -382,7 +379,7 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
     G.ntp = nR * (nR + 1) / 2;
     static size_t budget = gemm_lds_budget();
     G.st = gemm_st<T>(na, budget);
-    const uint64_t nwg = uint64_t(dext(d.ntime)) * dext(d.nchan);
+    const uint64_t nwg = 8 * uint64_t(tc_per_xcd(dext(d.ntime), dext(d.nchan)));
     RimeArgs<T> Ag = A;
     Ag.ct = 1;
     const size_t lds = gemm_lds_bytes<T>(na, G.st);

@@ -39,6 +39,7 @@ template <typename T> struct RimeArgs {
     int options;
     int ct, st, nbchunk, bchunk, nctile;
     int s_begin;              // first source the kernel sums (earlier ones are in vin)
+    uint32_t per_xcd;         // workgroup slots per XCD (grid = 8 per_xcd)
     const int32_t *blocks;    // pair-block plan (blocked kernels)
     int nblocks, pb, qb;
 };
@@ -118,8 +119,39 @@ static inline void fill_args(RimeArgs<T> &A, const mbx_dims *d, const mbx_inputs
     A.options = options;
     A.model = nullptr; A.chi2 = nullptr; A.ekb = nullptr; A.vin = nullptr; A.partial = nullptr;
     A.s_begin = 0;
+    A.per_xcd = 0;
     A.blocks = nullptr; A.nblocks = 0; A.pb = A.qb = 0;
     A.ct = A.st = A.nbchunk = A.bchunk = A.nctile = 1;
+}
+
+// Workgroup order. The dispatcher deals workgroups to the eight XCDs
+// round robin (workgroup x to XCD x mod 8), so slot w of XCD x is
+// workgroup x + 8 w and the workgroups an XCD runs together are
+// consecutive slots. Within an XCD's slots the channel varies fastest
+// over a group of kChanGroup channels, then time: concurrent workgroups
+// share kChanGroup channels, so their 32/64-byte visibility records fill
+// whole cache lines, and touch few beam planes (the L2 is per XCD).
+constexpr int kChanGroup = 4;
+
+__host__ __device__ inline uint32_t xcd_slots(uint64_t nwg) { return uint32_t((nwg + 7) / 8); }
+
+__device__ __forceinline__ uint64_t xcd_slot(uint32_t per_xcd) {
+    return uint64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+}
+
+// (t, c) of a one-channel workgroup over nt x nc; false past the grid.
+__device__ __forceinline__ bool tc_coords(uint32_t per_xcd, int nt, int nc, int &t, int &c) {
+    uint64_t w = xcd_slot(per_xcd);
+    const int ci = int(w % kChanGroup);
+    w /= kChanGroup;
+    t = int(w % uint64_t(nt));
+    c = int(w / uint64_t(nt)) * kChanGroup + ci;
+    return c < nc;
+}
+
+// Grid of tc_coords: 8 slots per XCD, padded.
+__host__ __device__ inline uint32_t tc_per_xcd(int nt, int nc) {
+    return xcd_slots(uint64_t(nt) * ((nc + kChanGroup - 1) / kChanGroup) * kChanGroup);
 }
 
 // LDS carve-up of one tile.
