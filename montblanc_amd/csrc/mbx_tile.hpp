@@ -539,11 +539,27 @@ __device__ __forceinline__ void pc_produce(const RimeArgs<T> &A, PcTile<T> &L, i
     }
 }
 
+// exp(-r2) and (1 + r2)^(-3/2): float uses the hardware exp2 and
+// reciprocal square root (about 1 ulp, well inside the float tolerance),
+// float64 the correctly rounded library functions.
+__device__ __forceinline__ float gauss_of(float r2) {
+    return __builtin_amdgcn_exp2f(-1.4426950408889634f * r2);
+}
+__device__ __forceinline__ double gauss_of(double r2) { return exp(-r2); }
+__device__ __forceinline__ float sersic_of(float r2) {
+    const float r = __builtin_amdgcn_rsqf(1.0f + r2);
+    return r * r * r;
+}
+__device__ __forceinline__ double sersic_of(double r2) {
+    const double den = 1.0 + r2;
+    return 1.0 / (den * sqrt(den));
+}
+
 template <typename T>
 __device__ __forceinline__ T gauss_f(T u, T v, T el, T em, T R, T k) {
     T x = (u * em - v * el) * k * R;
     T y = (u * el + v * em) * k;
-    return exp(-(x * x + y * y));
+    return gauss_of(x * x + y * y);
 }
 
 template <typename T>
@@ -551,8 +567,7 @@ __device__ __forceinline__ T sersic_f(T u, T v, T e1, T e2, T Rp, T k) {
     // Rp = R / (1 - e1^2 - e2^2), precomputed in phase 0
     T x = (u * (T(1) + e1) + v * e2) * k * Rp;
     T y = (u * e2 + v * (T(1) - e1)) * k * Rp;
-    T den = T(1) + x * x + y * y;
-    return T(1) / (den * sqrt(den));
+    return sersic_of(x * x + y * y);
 }
 
 // Epilogue of one visibility (t, b, c): M = G_p V G_q^H, flags,
