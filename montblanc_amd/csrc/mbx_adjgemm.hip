@@ -33,8 +33,10 @@
 namespace mbx {
 
 constexpr int kAdjThreads = 512;
-constexpr int kAdjMfmaWaves = 4;
-constexpr int kAdjMaxTiles = 2;      // output tiles per MFMA wave (nR * st / 8 <= 8)
+// Every wave takes one 16x16 output tile of the product (nR * st / 8 <= 8
+// tiles): two waves per SIMD hide each other's dependent-MFMA latency.
+constexpr int kAdjMfmaWaves = 8;
+constexpr int kAdjMaxTiles = 1;
 
 struct AdjGemmShape {
     int rows;    // 2 na rounded up to 16

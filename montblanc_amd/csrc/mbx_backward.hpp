@@ -124,7 +124,7 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
     for (int x = 0; x < 4; ++x) {
         const T n2 = S[x].re * S[x].re + S[x].im * S[x].im;
         if (n2 > T(0) && isfinite(n2)) {
-            const T inv = T(1) / sqrt(n2);
+            const T inv = rsqrt_acc(n2);
             const T g = Am[x] * inv;
             E.v[x] = {S[x].re * g, S[x].im * g};
             // dE = dS g + S dA / |S| - S A Re(conj(S) dS) / |S|^3

@@ -133,6 +133,19 @@ __host__ __device__ __forceinline__ uint32_t dext(const mbx_dim &x) {
 // B = [[I+Q, U+iV], [U-iV, I-Q]] * (f/rf)^alpha; sqrt(B) = (B + s 1)/t with
 // s = sqrt(det B), t = sqrt(trace B + 2 s); the matrix is left undivided
 // when t == 0 (then B == 0). CPUSolver.py:235-317, brightness.cuh:108-164.
+// 1 / sqrt(x) for x > 0: the hardware reciprocal square root, refined by
+// Newton steps y <- y (3 - x y^2) / 2 (one in float, two in float64, each
+// doubling the correct bits), cheaper than a divide by a square root.
+__device__ __forceinline__ float rsqrt_acc(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    return y * fmaf(-0.5f * x * y, y, 1.5f);
+}
+__device__ __forceinline__ double rsqrt_acc(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return y * fma(-0.5 * x * y, y, 1.5);
+}
+
 template <typename T>
 __device__ __forceinline__ J4<T> brightness_sqrt_scaled(T I, T Q, T U, T V, T p) {
     I *= p; Q *= p; U *= p; V *= p;

@@ -390,7 +390,7 @@ __device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, 
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         T n2 = acc[p].re * acc[p].re + acc[p].im * acc[p].im;
-        T g = (n2 > T(0) && isfinite(n2)) ? amp[p] / sqrt(n2) : amp[p];
+        T g = (n2 > T(0) && isfinite(n2)) ? amp[p] * rsqrt_acc(n2) : amp[p];
         E.v[p] = {acc[p].re * g, acc[p].im * g};
     }
     return E;
