@@ -255,12 +255,14 @@ def time_batched(steps=2):
     """Batched workload: 256 sky-model samples, source counts log-uniform
     in [10, 2000] (30% gaussian), against one shared observation of 64
     antennas, 50 times, 32 channels; float32 with float64 phase. chi^2 of
-    every sample, then chi^2 + gradient of every sample, each in one
-    persistent launch with autotuned tile shapes (the tuning runs in the
-    untimed warm-up)."""
+    every sample, then chi^2 + gradient of every sample. The autotuner
+    (run in the untimed warm-up) picks either one persistent launch over
+    all samples with tuned tile shapes, or the per-sample strategy (each
+    sample through the single-model matrix-core kernels, the sequence
+    replayed as one HIP graph); the choice is reported."""
     import numpy as np
     import torch
-    from montblanc_amd import batched, roofline
+    from montblanc_amd import autotune, batched
     from oracle import synth
 
     w = dict(synth.WORKLOADS['batched_64ant'])
@@ -284,15 +286,16 @@ def time_batched(steps=2):
     n_terms = sum(sum(c) for c in counts) * nt * nbl * w['nchan']
     out = {}
     for label, grads in (('chi2', False), ('chi2_gradient', True)):
-        batched.evaluate(b, gradients=grads)            # tunes and warms up
+        choice = autotune.batched_tuning(b, gradients=grads)
+        batched.evaluate(b, gradients=grads)            # warms up (graph capture)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             batched.evaluate(b, gradients=grads)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
-        out[label] = {'s_per_step': dt, 'terms_per_s': n_terms / dt}
-        print('# batched %s: %.1f ms, %.4g terms/s' % (label, dt * 1e3, n_terms / dt),
+        out[label] = {'s_per_step': dt, 'terms_per_s': n_terms / dt, 'tuning': choice}
+        print('# batched %s: %.1f ms, %.4g terms/s, %s' % (label, dt * 1e3, n_terms / dt, choice),
               file=sys.stderr, flush=True)
     out.update(workload='batched_64ant', samples=len(counts), sources=sum(sum(c) for c in counts),
                terms=n_terms, dtype='f32',

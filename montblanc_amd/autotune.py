@@ -65,9 +65,10 @@ def time_call(fn, repeats=3):
     return sorted(ts)[len(ts) // 2]
 
 
-def tune(key, candidates, run, use_cache=True):
+def tune(key, candidates, run, use_cache=True, measure=None):
     """Return the candidate with the smallest run(candidate) time; failing
-    candidates (MontblancHipError) are skipped."""
+    candidates (MontblancHipError) are skipped. measure(candidate), when
+    given, replaces the timing of run(candidate) by time_call."""
     if use_cache:
         if key in _MEM:
             return _MEM[key]
@@ -78,7 +79,7 @@ def tune(key, candidates, run, use_cache=True):
     best, best_t, times = None, None, {}
     for c in candidates:
         try:
-            t = time_call(lambda: run(c))
+            t = measure(c) if measure is not None else time_call(lambda: run(c))
         except _lib.MontblancHipError:
             continue
         times[json.dumps(c)] = t
@@ -98,13 +99,20 @@ def _probe_dims(arrays, counts, ntime_probe):
 
 
 def batched_tuning(batch, gradients=False, ntime_probe=2):
-    """Launch shape of the batched kernel for this batch's shape."""
+    """Launch shape (or strategy) of the batched evaluation for this batch.
+
+    Candidates are timed at two probe lengths of the time extent
+    (ntime_probe and 3*ntime_probe) and the time at the full extent is
+    extrapolated linearly: the per-sample strategy pays a fixed cost per
+    sample that a single short probe would weigh far above its share of
+    the full run."""
     from . import batched as bm
     a = batch.arrays
     na = a['uvw'].shape[1]
-    key = 'batched|%s|na%d|nbl%d|nchan%d|K%d|grad%d|%s' % (
-        rime.suffix(a), na, a['antenna1'].shape[1], a['frequency'].shape[0], batch.nsamples,
-        int(gradients), _device_name(batch.device))
+    ntime = a['antenna1'].shape[0]
+    key = 'batched|%s|na%d|nbl%d|nchan%d|nt%d|K%d|grad%d|%s' % (
+        rime.suffix(a), na, a['antenna1'].shape[1], a['frequency'].shape[0], ntime,
+        batch.nsamples, int(gradients), _device_name(batch.device))
     props = torch.cuda.get_device_properties(batch.device)
     cus = props.multi_processor_count
     cands = []
@@ -112,12 +120,27 @@ def batched_tuning(batch, gradients=False, ntime_probe=2):
         for cpi in (1, 2):
             for st in (0, 8):
                 cands.append({'workgroups': wg, 'channels_per_item': cpi, 'sources_per_tile': st})
+    # or every sample through the single-model matrix-core kernels, as one
+    # captured graph
+    cands.append({'strategy': 'per_sample'})
 
-    probe = _probe_dims(a, batch.totals, ntime_probe)
+    p1 = min(ntime, ntime_probe)
+    p2 = min(ntime, 3 * ntime_probe)
+    probes = [_probe_dims(a, batch.totals, p) for p in (p1, p2)]
 
-    def run(c):
-        bm.evaluate(batch, gradients=gradients, tune=c, dims=probe)
-    return tune(key, cands, run)
+    def run(c, d=probes[0]):
+        bm.evaluate(batch, gradients=gradients, tune=c, dims=d)
+
+    def measure(c):
+        t1 = time_call(lambda: run(c, probes[0]))
+        if p2 == p1:
+            return t1 * ntime / p1
+        t2 = time_call(lambda: run(c, probes[1]))
+        return t1 + (t2 - t1) * (ntime - p1) / (p2 - p1)
+    try:
+        return tune(key, cands, run, measure=measure)
+    finally:
+        bm.drop_probes(batch)
 
 
 def forward_tuning(arrays, counts, ntime_probe=4):

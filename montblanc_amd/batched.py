@@ -62,6 +62,7 @@ class Batch(object):
         self.X2 = torch.zeros(len(samples), dtype=torch.float64, device=device)
         self._grads = None
         self._ws = None
+        self._per_sample = {}
 
     @property
     def nsamples(self):
@@ -95,8 +96,92 @@ class Batch(object):
         return out
 
 
+class _PerSample(object):
+    """The per-sample strategy: every sample's chi^2 (and gradients) by the
+    single-model entry points -- point sources on the matrix cores, the
+    adjoint's product too -- one after another on one stream, the whole
+    sequence captured as one HIP graph and replayed. The alternative to the
+    persistent launch when samples are large enough to fill the device.
+
+    ntime_probe limits the time extent (autotuner probes)."""
+
+    def __init__(self, batch, gradients, use_weight_vector, sigma_sqrd, ntime_probe=None):
+        from . import graph
+        self.batch = batch
+        self.gradients = gradients
+        self.kw = dict(use_weight_vector=use_weight_vector, sigma_sqrd=sigma_sqrd)
+        a = batch.arrays
+        shared = {k: v for k, v in a.items() if k not in SOURCE_ROWS + ('gauss_shape', 'sersic_shape')}
+        # one launch plan and baseline map for every sample (same layout)
+        rime.pair_plan(a)
+        if gradients:
+            rime.baseline_map(a)
+        shared['_pair_plan'] = a['_pair_plan']
+        if '_baseline_map' in a:
+            shared['_baseline_map'] = a['_baseline_map']
+        ft = a['uvw'].dtype
+        ntime, nbl = a['antenna1'].shape
+        nchan = a['frequency'].shape[0]
+        dev = batch.device
+        ct = rime.complex_dtype(a)
+        self.out = dict(model_vis=torch.empty((ntime, nbl, nchan, 4), dtype=ct, device=dev),
+                        chi_sqrd_result=torch.empty((ntime, nbl, nchan), dtype=ft, device=dev))
+        if gradients:
+            self.out['ekb_vis'] = torch.empty((ntime, nbl, nchan, 4), dtype=ct, device=dev)
+            g = batch.grads_buffers()
+        self.samples = []
+        g0 = s0 = 0
+        o = batch.src_offset
+        for k, c in enumerate(batch.counts):
+            sa = dict(shared)
+            for n in SOURCE_ROWS:
+                sa[n] = a[n][o[k]:o[k + 1]]
+            sa['gauss_shape'] = a['gauss_shape'][:, g0:g0 + c[1]].contiguous()
+            sa['sersic_shape'] = a['sersic_shape'][:, s0:s0 + c[2]].contiguous()
+            dims = rime.make_dims(sa, c)
+            if ntime_probe is not None:
+                dims.ntime.upper_extent = min(dims.ntime.local_size, ntime_probe)
+            out = dict(self.out)
+            out['X2'] = batch.X2[k]
+            entry = dict(arrays=sa, counts=c, dims=dims, out=out, g0=g0, s0=s0)
+            if gradients:
+                entry['grads'] = dict(lm=g['lm'][o[k]:o[k + 1]], stokes=g['stokes'][o[k]:o[k + 1]],
+                                      alpha=g['alpha'][o[k]:o[k + 1]],
+                                      gauss_shape=torch.empty((3, c[1]), dtype=torch.float64, device=dev),
+                                      sersic_shape=torch.empty((3, c[2]), dtype=torch.float64, device=dev))
+            self.samples.append(entry)
+            g0 += c[1]
+            s0 += c[2]
+        self.graph = graph.Graphed(self._run, device=dev)
+
+    def _run(self):
+        b = self.batch
+        for e in self.samples:
+            rime.forward(e['arrays'], e['counts'], outputs=e['out'], keep_ekb=self.gradients,
+                         dims=e['dims'], **self.kw)
+            if self.gradients:
+                rime.adjoint(e['arrays'], e['counts'], e['out']['ekb_vis'], grads=e['grads'],
+                             dims=e['dims'], **self.kw)
+                g = b.grads_buffers()
+                c = e['counts']
+                if c[1]:
+                    g['gauss_shape'][:, e['g0']:e['g0'] + c[1]].copy_(e['grads']['gauss_shape'])
+                if c[2]:
+                    g['sersic_shape'][:, e['s0']:e['s0'] + c[2]].copy_(e['grads']['sersic_shape'])
+        return b.X2
+
+    def __call__(self):
+        return self.graph.replay()
+
+
+def drop_probes(batch):
+    """Release the per-sample graphs built for shortened (probe) dims."""
+    for k in [k for k in batch._per_sample if k[-1] is not None]:
+        del batch._per_sample[k]
+
+
 def tuning_struct(tune):
-    if tune is None:
+    if tune is None or tune.get('strategy') == 'per_sample':
         return None
     t = _lib.mbx_batch_tuning(int(tune.get('workgroups', 0)), int(tune.get('channels_per_item', 0)),
                               int(tune.get('sources_per_tile', 0)), 0)
@@ -108,12 +193,27 @@ def evaluate(batch, gradients=False, use_weight_vector=False, sigma_sqrd=None, t
     """Per-sample chi^2 (float64 device tensor [K]) and, with gradients=True,
     a dict of float64 gradient tensors over the concatenated sources.
 
-    tune: a dict (workgroups, channels_per_item, sources_per_tile), None for
-    the library defaults, or 'auto' for the cached autotuner choice.
+    tune: a dict (workgroups, channels_per_item, sources_per_tile) for the
+    persistent launch, {'strategy': 'per_sample'} for the per-sample graph
+    (which runs on the current stream; ``stream`` applies to the persistent
+    launch), None for the library defaults, or 'auto' for the cached
+    autotuner choice.
     """
     if tune == 'auto':
         from . import autotune
         tune = autotune.batched_tuning(batch, gradients=gradients)
+    if isinstance(tune, dict) and tune.get('strategy') == 'per_sample':
+        key = ('per_sample', bool(gradients), bool(use_weight_vector), sigma_sqrd,
+               None if dims is None else int(dims.ntime.upper_extent))
+        ps = batch._per_sample.get(key)
+        if ps is None:
+            ps = _PerSample(batch, gradients, use_weight_vector, sigma_sqrd,
+                            None if dims is None else int(dims.ntime.upper_extent))
+            batch._per_sample[key] = ps
+        ps()
+        if gradients:
+            return batch.X2, batch.grads_buffers()
+        return batch.X2
     a = batch.arrays
     d = batch.dims() if dims is None else dims
     ts = tuning_struct(tune)

@@ -262,13 +262,13 @@ def alloc_grads(arrays, counts):
 
 
 def adjoint(arrays, counts, ekb_vis, grad_scale=1.0, use_weight_vector=False, sigma_sqrd=None,
-            grads=None, stream=None):
+            grads=None, stream=None, dims=None):
     """d chi^2 / d(lm, stokes, alpha, gauss_shape, sersic_shape, G_term) times
     grad_scale, from the forward's pre-gain sums ``ekb_vis``. Returns a dict
     of float64 tensors (G_term complex128: d/dRe + i d/dIm)."""
     dev = arrays['uvw'].device
     grads = alloc_grads(arrays, counts) if grads is None else grads
-    d = make_dims(arrays, counts)
+    d = make_dims(arrays, counts) if dims is None else dims
     bmap = baseline_map(arrays)
     ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
     g = _lib.mbx_grads(**{n: _raw(grads.get(n)) for n in _lib.GRAD_NAMES})
