@@ -330,6 +330,301 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     }
 }
 
+// ---------------------------------------------------------------------
+// Pipelined point pass. Producer waves and matrix-core waves work on
+// different source tiles at once, so the product overlaps the vector work
+// instead of following it:
+//   phase p (between two barriers)
+//     producers (waves 0-7): X of tile p (Jones only) into X buffer p % 2;
+//       the backward of tile p - 2 (forward state recomputed, Jb from Jb
+//       buffer p % 2) into the gradient sums; the gradient window that
+//       ended with tile p - 3 to global memory;
+//     matrix waves (one per 16-row block of W): Jb of tile p - 1 =
+//       W X[(p - 1) % 2] into Jb buffer (p - 1) % 2; the first of them
+//       also stages the parameters of tile p + 1 (slot (p + 1) % 4).
+// W stays in the matrix waves' registers for the whole (t, c): each holds
+// the A operand of its 16-row block, gathered from Vb through the
+// baseline map. X columns are n = j st + s (j the Jones column, s the
+// source in the tile); X[row][n] sits at (row / 4) XS + 4 n + row % 4,
+// the row-block stride XS = 8 st + 4 padded so that producers writing
+// different antennas hit different LDS banks. Jb is kept as eight planes
+// (Re/Im of the four Jones entries) of [antenna][source].
+
+template <typename T> struct AdjPoint {
+    static constexpr int kMaxRb = sizeof(T) == 8 ? 4 : 8;   // 2 na <= 16 kMaxRb
+    static constexpr int kMaxCb = 2;                         // st <= 16
+    static constexpr int kProducers = 512;
+    static constexpr int kThreads = kProducers + 64 * kMaxRb;
+    static constexpr int kTaps = sizeof(T) == 4 ? 4 : 2;     // beam taps in flight
+    static constexpr int kParams = 16;                       // per source and slot
+    // which role stages the next tile's parameters: in float64 the matrix
+    // waves (registers to spare), in float the producers (the matrix
+    // waves' W leaves no room)
+    static constexpr bool kParamsOnMatrix = sizeof(T) == 8;
+};
+
+struct AdjPointShape {
+    int rows;    // 2 na rounded up to 16
+    int nrb;     // rows / 16: matrix waves
+    int st;      // sources per tile: 8 or 16, st * na <= 512
+    int xs;      // X row-block stride (T)
+    int wt;      // tiles per gradient window
+};
+
+static __host__ __device__ inline size_t adjp_align(size_t b) { return (b + 31) & ~size_t(31); }
+
+template <typename T>
+struct AdjPointLds {
+    size_t params, x, jb, sacc, total;
+    __host__ __device__ AdjPointLds(int na, const AdjPointShape &S) {
+        params = adjg_setup_bytes(na);
+        x = params + adjp_align(sizeof(double) * 4 * AdjPoint<T>::kParams * S.st);
+        jb = x + adjp_align(sizeof(T) * 2 * 2 * (S.rows / 4) * S.xs);
+        sacc = jb + adjp_align(sizeof(T) * 2 * 8 * na * S.st);
+        total = sacc + sizeof(double) * 2 * S.wt * S.st * 7;
+    }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(AdjPoint<T>::kThreads)
+k_adj_point(AdjArgs<T> A, AdjPointShape S) {
+    typedef AdjPoint<T> C;
+    typedef typename Mfma<T>::acc_t acc_t;
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    const RimeArgs<T> &R = A.R;
+    const mbx_dims &d = R.d;
+    const int NA = d.na.local_size;
+    const uint64_t NT = d.ntime.local_size, NC = d.nchan.local_size, NBL = d.nbl.local_size;
+    const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan), nb_ext = dext(d.nbl);
+    const int tid = threadIdx.x;
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
+    const int rows = S.rows, st = S.st, xs = S.xs, nth = C::kProducers + 64 * S.nrb;
+    const AdjPointLds<T> L(NA, S);
+
+    double *Luvw = reinterpret_cast<double *>(smem_raw);
+    double *Lpa = Luvw + 3 * NA;
+    double *Lant = Lpa + 2 * NA;
+    double *Lch = Lant + 4 * NA;
+    double *Lprm = reinterpret_cast<double *>(smem_raw + L.params);   // [4][kParams][st]
+    T *Xbuf = reinterpret_cast<T *>(smem_raw + L.x);                  // [2][re, im][rows/4][xs]
+    T *Jbuf = reinterpret_cast<T *>(smem_raw + L.jb);                 // [2][8][na][st]
+    double *sacc = reinterpret_cast<double *>(smem_raw + L.sacc);     // [2][wt st][7]
+    const int xplane = (rows / 4) * xs, jplane = 8 * NA * st, wsrc = S.wt * st;
+    auto prm = [&](int tile, int f) { return Lprm + ((tile & 3) * C::kParams + f) * st; };
+
+    tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, nth);
+    stage_antennas<T>(R, Lant, t, c, tid, nth);
+    // X rows past 2 na read as zero in every tile (producers never write them)
+    for (int i = tid; i < 2 * 2 * xplane; i += nth) Xbuf[i] = T(0);
+    for (int i = tid; i < 2 * wsrc * 7; i += nth) sacc[i] = 0.0;
+
+    const int np = dext(d.npsrc);
+    const int ntiles = (np + st - 1) / st;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool producer = wave < C::kProducers / 64;
+    const int i16 = lane & 15, g = lane >> 4;
+    auto tile_n = [&](int it) { return min(st, np - it * st); };
+
+    // parameters of tile `it` by the first matrix wave: l, m, n - 1, I, Q,
+    // U, V, P = (f / f_ref)^alpha, sqrt(B) (8 reals)
+    auto stage_params = [&](int it, int si) {
+        if (it >= ntiles || si >= tile_n(it)) return;
+        const int s = it * st + si;
+        double l0, m0, nm1;
+        const J4<T> B = source_sqrtb<T>(R, Lch, t, s, l0, m0, nm1);
+        const T *Sk = R.stokes + (uint64_t(s) * NT + t) * 4;
+        double v[C::kParams];
+        v[0] = l0; v[1] = m0; v[2] = nm1;
+        v[3] = Sk[0]; v[4] = Sk[1]; v[5] = Sk[2]; v[6] = Sk[3];
+        v[7] = exp_t<T>(double(R.alpha[uint64_t(s) * NT + t]) * Lch[4]);
+#pragma unroll
+        for (int x = 0; x < 4; ++x) { v[8 + 2 * x] = B.v[x].re; v[9 + 2 * x] = B.v[x].im; }
+#pragma unroll
+        for (int f = 0; f < C::kParams; ++f) prm(it, f)[si] = v[f];
+    };
+
+    auto flush = [&](int done, int wbase) {
+        const int ns = min(np, (done + 1) * st) - wbase * st;
+        double *sa = sacc + ((done / S.wt) & 1) * wsrc * 7;
+        const double gs = A.gscale;
+        for (int i = tid; i < ns; i += C::kProducers) {
+            const uint64_t s = uint64_t(wbase) * st + i;
+            double *v = sa + i * 7;
+            if (A.g_lm) {
+                atomicAdd(A.g_lm + 2 * s, gs * v[0]);
+                atomicAdd(A.g_lm + 2 * s + 1, gs * v[1]);
+            }
+            if (A.g_stokes)
+                for (int x = 0; x < 4; ++x) atomicAdd(A.g_stokes + (s * NT + t) * 4 + x, gs * v[2 + x]);
+            if (A.g_alpha) atomicAdd(A.g_alpha + s * NT + t, gs * v[6]);
+#pragma unroll
+            for (int x = 0; x < 7; ++x) v[x] = 0.0;
+        }
+    };
+
+    // The two roles run separate loops with the same barriers, so that the
+    // matrix waves' W registers are not live in the producers' code.
+    if (producer) {
+        if (!C::kParamsOnMatrix && wave == 0) stage_params(0, lane);
+        __syncthreads();
+        int wbase = 0;                 // first tile of the window flushed next
+        for (int p = 0; p < ntiles + 2; ++p) {
+            // ---- X of tile p: item (si, a), consecutive threads take
+            // consecutive sources
+            if (p < ntiles) {
+                const int nst = tile_n(p);
+                if (tid < nst * NA) {
+                    const int si = tid % nst, a = tid / nst;
+                    J4<T> B;
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) B.v[x] = {T(prm(p, 8 + 2 * x)[si]), T(prm(p, 9 + 2 * x)[si])};
+                    const J4<T> J = staged_jones<T, C::kTaps>(R, Luvw, Lpa, Lant, Lch, c, a, prm(p, 0)[si],
+                                                               prm(p, 1)[si], prm(p, 2)[si], B);
+                    T *xr = Xbuf + (p & 1) * 2 * xplane, *xi = xr + xplane;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) {
+                            const int row = 2 * a + k, o = (row >> 2) * xs + 4 * (j * st + si) + (row & 3);
+                            xr[o] = J.v[2 * k + j].re;
+                            xi[o] = J.v[2 * k + j].im;
+                        }
+                }
+            }
+            // ---- backward of tile p - 2
+            if (p >= 2) {
+                const int it = p - 2, nst = tile_n(it);
+                if (tid < nst * NA) {
+                    const int si = tid % nst, a = tid / nst;
+                    const double *an = Lant + 4 * a;
+                    const T *jb = Jbuf + (it & 1) * jplane + a * st + si;
+                    const int js = NA * st;
+                    auto Jb = [&](int x) { return cx<T>{jb[(2 * x) * js], jb[(2 * x + 1) * js]}; };
+                    double gr[7];
+                    item_adjoint<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2], an[3], c,
+                                    Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si], T(prm(it, 3)[si]),
+                                    T(prm(it, 4)[si]), T(prm(it, 5)[si]), T(prm(it, 6)[si]),
+                                    T(prm(it, 7)[si]), Jb, gr);
+                    double *sa = sacc + ((it / S.wt) & 1) * wsrc * 7 + ((it % S.wt) * st + si) * 7;
+#pragma unroll
+                    for (int x = 0; x < 7; ++x) atomicAdd(sa + x, gr[x]);
+                }
+            }
+            if (!C::kParamsOnMatrix && wave == 0) stage_params(p + 1, lane);
+            // ---- the window that ended with tile p - 3 (summed by phase p - 1)
+            const int done = p - 3;
+            if (done >= 0 && done % S.wt == S.wt - 1) {
+                flush(done, wbase);
+                wbase = done + 1;
+            }
+            __syncthreads();
+        }
+        // the last window (summed in the last phase)
+        if (ntiles > 0 && wbase < ntiles) flush(ntiles - 1, wbase);
+    } else {
+        // W rows of this matrix wave's block: lane (i16, g) supplies
+        // A[row][k] = W[16 rb + arow(i16)][16 kb + 4 g + m] for m = 0..3
+        T Wr[C::kMaxRb][4], Wi[C::kMaxRb][4];
+        const int rb = wave - C::kProducers / 64;
+        {
+            const int wrow = 16 * rb + Mfma<T>::arow(i16);
+            const int p = wrow >> 1, i = wrow & 1;
+#pragma unroll
+            for (int kb = 0; kb < C::kMaxRb; ++kb)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int col = 16 * kb + 4 * g + m, q = col >> 1, k = col & 1;
+                    T wr = T(0), wi = T(0);
+                    if (kb < S.nrb && p < NA && q < NA) {
+                        const int e = A.bmap[p * NA + q];
+                        const int b = e & kBlMask;
+                        if (e >= 0 && b < nb_ext) {
+                            const cx<T> *v = A.vbar + ((uint64_t(t) * NBL + b) * NC + c) * 4;
+                            if (e & MBX_BL_AUTO) {                 // Vb + Vb^H
+                                const cx<T> a = v[2 * i + k], h = v[2 * k + i];
+                                wr = a.re + h.re; wi = a.im - h.im;
+                            } else if (e & MBX_BL_REVERSED) {      // (Vb^H)[i][k]
+                                const cx<T> h = v[2 * k + i];
+                                wr = h.re; wi = -h.im;
+                            } else {
+                                const cx<T> a = v[2 * i + k];
+                                wr = a.re; wi = a.im;
+                            }
+                        }
+                    }
+                    Wr[kb][m] = wr; Wi[kb][m] = wi;
+                }
+        }
+        if (C::kParamsOnMatrix && rb == 0) stage_params(0, lane);
+        __syncthreads();
+        for (int p = 0; p < ntiles + 2; ++p) {
+            // ---- Jb of tile p - 1 on the matrix cores
+            if (p >= 1 && p - 1 < ntiles) {
+                const int it = p - 1, nst = tile_n(it);
+                const T *xr = Xbuf + (it & 1) * 2 * xplane, *xi = xr + xplane;
+                T *jb = Jbuf + (it & 1) * jplane;
+#pragma unroll
+                for (int cb = 0; cb < C::kMaxCb; ++cb) {
+                    if (cb >= st / 8) break;
+                    acc_t ore = acc_t{T(0), T(0), T(0), T(0)}, oim = ore;
+                    const int xn = 16 * cb + i16;
+#pragma unroll
+                    for (int kb = 0; kb < C::kMaxRb; ++kb) {
+                        if (kb >= S.nrb) break;
+                        const int o = (4 * kb + g) * xs + 4 * xn;
+                        T x4[4], y4[4];
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) { x4[m] = xr[o + m]; y4[m] = xi[o + m]; }
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) {
+                            ore = Mfma<T>::mma(Wr[kb][m], x4[m], ore);
+                            ore = Mfma<T>::mma(Wi[kb][m], -y4[m], ore);
+                            oim = Mfma<T>::mma(Wi[kb][m], x4[m], oim);
+                            oim = Mfma<T>::mma(Wr[kb][m], y4[m], oim);
+                        }
+                    }
+                    const int j = xn / st, si = xn % st;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * rb + out_row(lane, r), a = row >> 1, i = row & 1;
+                        if (a < NA && si < nst) {
+                            const int comp = 2 * (2 * i + j);
+                            jb[(comp * NA + a) * st + si] = ore[r];
+                            jb[((comp + 1) * NA + a) * st + si] = oim[r];
+                        }
+                    }
+                }
+            }
+            if (C::kParamsOnMatrix && rb == 0) stage_params(p + 1, lane);
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T>
+static bool adjp_shape(const mbx_dims &d, AdjPointShape &S, size_t &lds) {
+    typedef AdjPoint<T> C;
+    const int na = d.na.local_size;
+    if (na <= 0) return false;
+    S.rows = ((2 * na + 15) / 16) * 16;
+    S.nrb = S.rows / 16;
+    if (S.nrb > C::kMaxRb) return false;
+    S.st = 16 * na <= C::kProducers ? 16 : (8 * na <= C::kProducers ? 8 : 0);
+    if (S.st == 0) return false;
+    S.xs = 8 * S.st + 4;
+    static size_t budget = gemm_lds_budget();
+    // the widest gradient window that fits, up to 32 tiles
+    S.wt = 1;
+    if (AdjPointLds<T>(na, S).total > budget) return false;
+    while (S.wt < 32) {
+        ++S.wt;
+        if (AdjPointLds<T>(na, S).total > budget) { --S.wt; break; }
+    }
+    lds = AdjPointLds<T>(na, S).total;
+    return true;
+}
+
 template <typename T>
 static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds) {
     const int na = d.na.local_size;
@@ -365,7 +660,13 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
     AdjArgs<T> Ag = A;
     Ag.R.ct = 1;
     if (dext(d.npsrc) > 0) {
-        hipLaunchKernelGGL((k_adj_gemm<T, false>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
+        AdjPointShape PS;
+        size_t plds;
+        if (adjp_shape<T>(d, PS, plds))
+            hipLaunchKernelGGL((k_adj_point<T>), dim3(nwg), dim3(AdjPoint<T>::kProducers + 64 * PS.nrb),
+                               plds, stream, Ag, PS);
+        else
+            hipLaunchKernelGGL((k_adj_gemm<T, false>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
         int r = hip_status(hipGetLastError(), "adjoint point gemm launch");
         if (r) return r;
     }

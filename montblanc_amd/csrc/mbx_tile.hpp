@@ -207,28 +207,36 @@ __device__ __forceinline__ double reduce_half_pi(double phi, int &quadrant) {
     return fma(-q, pio2_lo, fma(-q, pio2_hi, phi));
 }
 
+// A float64 constant placed in a scalar register pair where it is used:
+// keeps loop-invariant polynomial coefficients from being hoisted into
+// vector registers (and spilled) in register-bound kernels.
+__device__ __forceinline__ double sconst(double v) {
+    __asm__ volatile("" : "+s"(v));
+    return v;
+}
+
 __device__ __forceinline__ void taylor_sincos(double y, double *s, double *c) {
     const double y2 = y * y;
     // sin y = y (1 - y^2/3! + y^4/5! - ... - y^16/17!)
-    double ps = -1.0 / 355687428096000.0;            // -1/17!
-    ps = fma(ps, y2, 1.0 / 1307674368000.0);         //  1/15!
-    ps = fma(ps, y2, -1.0 / 6227020800.0);           // -1/13!
-    ps = fma(ps, y2, 1.0 / 39916800.0);              //  1/11!
-    ps = fma(ps, y2, -1.0 / 362880.0);               // -1/9!
-    ps = fma(ps, y2, 1.0 / 5040.0);                  //  1/7!
-    ps = fma(ps, y2, -1.0 / 120.0);                  // -1/5!
-    ps = fma(ps, y2, 1.0 / 6.0);                     //  1/3!
+    double ps = sconst(-1.0 / 355687428096000.0);            // -1/17!
+    ps = fma(ps, y2, sconst(1.0 / 1307674368000.0));         //  1/15!
+    ps = fma(ps, y2, sconst(-1.0 / 6227020800.0));           // -1/13!
+    ps = fma(ps, y2, sconst(1.0 / 39916800.0));              //  1/11!
+    ps = fma(ps, y2, sconst(-1.0 / 362880.0));               // -1/9!
+    ps = fma(ps, y2, sconst(1.0 / 5040.0));                  //  1/7!
+    ps = fma(ps, y2, sconst(-1.0 / 120.0));                  // -1/5!
+    ps = fma(ps, y2, sconst(1.0 / 6.0));                     //  1/3!
     *s = fma(-y * y2, ps, y);
     // cos y = 1 - y^2/2! + y^4/4! - ... + y^18/18!
-    double pc = 1.0 / 6402373705728000.0;            //  1/18!
-    pc = fma(pc, y2, -1.0 / 20922789888000.0);       // -1/16!
-    pc = fma(pc, y2, 1.0 / 87178291200.0);           //  1/14!
-    pc = fma(pc, y2, -1.0 / 479001600.0);            // -1/12!
-    pc = fma(pc, y2, 1.0 / 3628800.0);               //  1/10!
-    pc = fma(pc, y2, -1.0 / 40320.0);                // -1/8!
-    pc = fma(pc, y2, 1.0 / 720.0);                   //  1/6!
-    pc = fma(pc, y2, -1.0 / 24.0);                   // -1/4!
-    pc = fma(pc, y2, 0.5);                           //  1/2!
+    double pc = sconst(1.0 / 6402373705728000.0);            //  1/18!
+    pc = fma(pc, y2, sconst(-1.0 / 20922789888000.0));       // -1/16!
+    pc = fma(pc, y2, sconst(1.0 / 87178291200.0));           //  1/14!
+    pc = fma(pc, y2, sconst(-1.0 / 479001600.0));            // -1/12!
+    pc = fma(pc, y2, sconst(1.0 / 3628800.0));               //  1/10!
+    pc = fma(pc, y2, sconst(-1.0 / 40320.0));                // -1/8!
+    pc = fma(pc, y2, sconst(1.0 / 720.0));                   //  1/6!
+    pc = fma(pc, y2, sconst(-1.0 / 24.0));                   // -1/4!
+    pc = fma(pc, y2, sconst(0.5));                           //  1/2!
     *c = fma(-y2, pc, 1.0);
 }
 

@@ -75,6 +75,31 @@ def test_adjoint_autocorrelations_and_single_channel(dtype):
         assert rel(got[name], ref[name]) < TOL[dtype], name
 
 
+@pytest.mark.parametrize('dtype,na,counts', [
+    (np.float64, 9, (40, 3, 0)),      # three point tiles of 16, pipelined
+    (np.float64, 27, (300, 0, 0)),    # 19 tiles: several gradient windows
+    (np.float64, 33, (20, 0, 0)),     # 2 na > 64: the single-phase kernel
+    (np.float32, 27, (35, 2, 0)),
+    (np.float32, 64, (70, 0, 1)),     # tiles of 8 sources, 8 matrix waves
+])
+def test_adjoint_many_point_tiles(dtype, na, counts):
+    """Point sources spread over many source tiles: the pipelined adjoint
+    (X of tile p, product of tile p - 1 and backward of tile p - 2 in one
+    phase) and its gradient windows."""
+    a = problem(na, 2, 3, counts, seed=na + counts[0])
+    ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts, sigma_sqrd=1.3)
+    got, _ = gpu_grads(a, counts, dtype, False)
+    # float32: each gradient sums na * ntime * nchan item terms of both signs
+    # per source, so its rounding grows with the problem (about 1e-4 here)
+    tol = TOL[dtype] if dtype == np.float64 else 3e-4
+    for name in NAMES:
+        if ref[name] is None or ref[name].size == 0:
+            continue
+        e = rel(got[name], ref[name])
+        print(name, e)
+        assert e < tol, (name, e)
+
+
 def test_autograd_function_and_gpu_finite_differences():
     """The torch.autograd.Function path gives the same gradients as the
     adjoint, and they match central differences (h = 1e-7 relative) of the
