@@ -54,7 +54,24 @@ static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw) {
            + 2 * size_t(2 * st) * rows * sizeof(T)                      // X planes (then Jb)
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
            + size_t(st) * 4 * sizeof(double)                            // l, m, n - 1
-           + size_t(st) * 20 * sizeof(T);                               // shape, I Q U V P, sqrt(B)
+           + size_t(st) * 24 * sizeof(T)                                // shape, I Q U V P, sqrt(B)
+           + (2 * size_t(na) * st + 2 * size_t(na)) * sizeof(T);        // shape coordinates, u, v
+}
+
+// Shape factors of a pair from r2 = dx^2 + dy^2: gaussian exp(-r2) (float:
+// the hardware exp2), sersic (1 + r2)^(-3/2) with id = 1 / (1 + r2).
+static __device__ __forceinline__ float gauss_factor(float r2) {
+    return __builtin_amdgcn_exp2f(-1.4426950408889634f * r2);
+}
+static __device__ __forceinline__ double gauss_factor(double r2) { return exp(-r2); }
+static __device__ __forceinline__ float sersic_factor(float r2, float &id) {
+    const float rs = __builtin_amdgcn_rsqf(1.0f + r2);
+    id = rs * rs;
+    return id * rs;
+}
+static __device__ __forceinline__ double sersic_factor(double r2, double &id) {
+    id = 1.0 / (1.0 + r2);
+    return id * sqrt(id);
 }
 
 template <typename T, bool SHAPED>
@@ -95,14 +112,20 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     J4<T> *Jb = reinterpret_cast<J4<T> *>(Xr); // [st][NA], written over X
     double *sacc = reinterpret_cast<double *>(Xi + 2 * st * rows);   // [st][10]
     double *Lsd = sacc + S.sw * 10;                                 // [st][4]: l, m, n - 1
-    T *Lsh = reinterpret_cast<T *>(Lsd + st * 4);                   // [st][4]: shape
-    T *Lst = Lsh + st * 4;                                          // [st][16]: IQUV P . sqrt(B)
+    T *Lsh = reinterpret_cast<T *>(Lsd + st * 4);                   // [st][8]: shape record
+    T *Lst = Lsh + st * 8;                                          // [st][16]: IQUV P . sqrt(B)
+    // shaped pass: shape coordinates x, y of (antenna, source) [2][na][st]
+    // and the antennas' u, v [2][na]
+    T *Lxy = Lst + st * 16;
+    T *Luv = Lxy + 2 * NA * st;
 
     tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kAdjThreads);
     stage_antennas<T>(R, Lant, t, c, tid, kAdjThreads);
     for (int i = tid; i < 2 * rows * rows; i += kAdjThreads) Wr[i] = T(0);
     for (int i = tid; i < S.sw * 10; i += kAdjThreads) sacc[i] = 0.0;
     __syncthreads();
+    if (SHAPED)
+        for (int a = tid; a < NA; a += kAdjThreads) { Luv[a] = T(Luvw[3 * a]); Luv[NA + a] = T(Luvw[3 * a + 1]); }
     // W from this (t, c)'s visibility adjoints
     for (int b = tid; b < nb_ext; b += kAdjThreads) {
         const int p = min(max(R.a1[uint64_t(t) * NBL + b], 0), NA - 1);
@@ -149,6 +172,25 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             o[0] = St[0]; o[1] = St[1]; o[2] = St[2]; o[3] = St[3];
             o[4] = exp_t<T>(double(R.alpha[uint64_t(s0 + si) * NTT + t]) * Lch[4]);
             reinterpret_cast<J4<T> *>(o + 8)[0] = B;
+            if (SHAPED) {
+                // shape record: gaussian 1, k, k R, el, em, R (k = gauss_scale f);
+                // sersic 2, k R', 2 e1 / den, 2 e2 / den, k / den, e1, e2
+                // (k = 2 pi f / C, den = 1 - e1^2 - e2^2, R' = R / den)
+                const int s = s0 + si;
+                T *h = Lsh + 8 * si;
+                if (s < np + ng) {
+                    const int g = s - np;
+                    const double el = R.gauss[g], em = R.gauss[g + NGS], Rg = R.gauss[g + 2 * NGS];
+                    const double k = R.gauss_scale * Lch[3];
+                    h[0] = T(1); h[1] = T(k); h[2] = T(k * Rg); h[3] = T(el); h[4] = T(em); h[5] = T(Rg);
+                } else {
+                    const int g = s - np - ng;
+                    const double e1 = R.sersic[g], e2 = R.sersic[g + NSS], Rs = R.sersic[g + 2 * NSS];
+                    const double den = 1.0 - e1 * e1 - e2 * e2, k = R.two_pi_over_c * Lch[3];
+                    h[0] = T(2); h[1] = T(k * Rs / den); h[2] = T(2.0 * e1 / den); h[3] = T(2.0 * e2 / den);
+                    h[4] = T(k / den); h[5] = T(e1); h[6] = T(e2);
+                }
+            }
         }
         __syncthreads();
         // ---- X of the tile (columns of sources past nst are zero). Each
@@ -173,26 +215,29 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     Xr[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].re;
                     Xi[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].im;
                 }
+            if (SHAPED) {
+                // shape coordinates, linear in the antenna's (u, v):
+                // f_pq = F((x_q - x_p)^2 + (y_q - y_p)^2)
+                const T *h = Lsh + 8 * isi;
+                const double u = Luvw[3 * ia], v = Luvw[3 * ia + 1];
+                double x, y;
+                if (h[0] == T(1)) {
+                    x = (u * double(h[4]) - v * double(h[3])) * double(h[2]);
+                    y = (u * double(h[3]) + v * double(h[4])) * double(h[1]);
+                } else {
+                    const double e1 = h[5], e2 = h[6];
+                    x = (u * (1.0 + e1) + v * e2) * double(h[1]);
+                    y = (u * e2 + v * (1.0 - e1)) * double(h[1]);
+                }
+                Lxy[ia * st + isi] = T(x);
+                Lxy[(NA + ia) * st + isi] = T(y);
+            }
         }
         for (int i = tid; i < 2 * st * rows; i += kAdjThreads) {
             const int n = SHAPED ? i % (2 * st) : (i >> 2) % (2 * st);
             const int r = SHAPED ? i / (2 * st) : (i / (8 * st)) * 4 + (i & 3);
             if (n >= 2 * nst || r >= 2 * NA) { Xr[i] = T(0); Xi[i] = T(0); }
         }
-        if (SHAPED)
-            for (int si = tid; si < nst; si += kAdjThreads) {
-                const int s = s0 + si;
-                T *o = Lsh + 4 * si;
-                if (s < np + ng) {
-                    const int g = s - np;
-                    o[0] = R.gauss[g]; o[1] = R.gauss[g + NGS]; o[2] = R.gauss[g + 2 * NGS]; o[3] = T(1);
-                } else {
-                    const int g = s - np - ng;
-                    const T e1 = R.sersic[g], e2 = R.sersic[g + NSS];
-                    o[0] = e1; o[1] = e2; o[2] = R.sersic[g + 2 * NSS] / (T(1) - e1 * e1 - e2 * e2);
-                    o[3] = T(2);
-                }
-            }
         __syncthreads();
         // ---- Jb = sum_q f W_pq J_q
         if (!SHAPED) {
@@ -245,24 +290,26 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 }
             }
         } else {
-            // one (source, antenna p) per thread: st * NA <= threads
+            // one (source, antenna p) per thread: st * NA <= threads. The
+            // shape-factor adjoint of a pair is the same in both orders
+            // (f and its derivatives are even in the pair's (u, v), and
+            // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): it is formed for
+            // q > p only and counted twice.
             const bool mine = tid < nst * NA;
             const int si = mine ? tid % nst : 0, p = mine ? tid / nst : 0;
             J4<T> acc = jzero<T>();
             T sh0 = T(0), sh1 = T(0), sh2 = T(0);
             if (mine) {
-                const T *sp = Lsh + 4 * si;
-                const int ty = int(sp[3]);
-                const T kf = T((ty == 1 ? R.gauss_scale : R.two_pi_over_c) * Lch[3]);
+                const T *h = Lsh + 8 * si;
                 J4<T> Jp;
 #pragma unroll
                 for (int k = 0; k < 2; ++k)
 #pragma unroll
                     for (int j = 0; j < 2; ++j)
                         Jp.v[2 * k + j] = {Xr[xi_(2 * si + j, 2 * p + k)], Xi[xi_(2 * si + j, 2 * p + k)]};
-                const double up = Luvw[3 * p], vp = Luvw[3 * p + 1];
-#pragma unroll 1
-                for (int q = 0; q < NA; ++q) {
+                const T *Lx = Lxy + si, *Ly = Lxy + NA * st + si;
+                const T xp = Lx[p * st], yp = Ly[p * st], up = Luv[p], vp = Luv[NA + p];
+                auto pair_y = [&](int q) {
                     J4<T> Wq, Jq;
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
@@ -271,17 +318,56 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                             Wq.v[2 * i + k] = {Wr[wi_(2 * p + i, 2 * q + k)], Wi[wi_(2 * p + i, 2 * q + k)]};
                             Jq.v[2 * i + k] = {Xr[xi_(2 * si + k, 2 * q + i)], Xi[xi_(2 * si + k, 2 * q + i)]};
                         }
-                    const J4<T> Y = jmul(Wq, Jq);
-                    T df[3];
-                    const T f = shape_factor_grad<T>(ty, T(Luvw[3 * q] - up), T(Luvw[3 * q + 1] - vp),
-                                                     sp[0], sp[1], sp[2], kf, df);
+                    return jmul(Wq, Jq);
+                };
+                auto accumulate = [&](T f, const J4<T> &Y) {
 #pragma unroll
                     for (int x = 0; x < 4; ++x) {
                         acc.v[x].re = fma(f, Y.v[x].re, acc.v[x].re);
                         acc.v[x].im = fma(f, Y.v[x].im, acc.v[x].im);
                     }
-                    const T fb = jdot_re(Jp, Y);
-                    sh0 = fma(fb, df[0], sh0); sh1 = fma(fb, df[1], sh1); sh2 = fma(fb, df[2], sh2);
+                };
+                if (h[0] == T(1)) {
+                    // gaussian: f = exp(-r2); d f / d(el, em, R) = -2 f k
+                    // (dy du - R dx dv, R dx du + dy dv, dx (du em - dv el))
+                    const T k = h[1], el = h[3], em = h[4], Rg = h[5];
+#pragma unroll 1
+                    for (int q = 0; q < NA; ++q) {
+                        const J4<T> Y = pair_y(q);
+                        const T dx = Lx[q * st] - xp, dy = Ly[q * st] - yp;
+                        const T f = gauss_factor(fma(dx, dx, dy * dy));
+                        accumulate(f, Y);
+                        if (q > p) {
+                            const T du = Luv[q] - up, dv = Luv[NA + q] - vp;
+                            const T tq = T(-4) * f * k * jdot_re(Jp, Y);
+                            sh0 = fma(tq, dy * du - Rg * dx * dv, sh0);
+                            sh1 = fma(tq, Rg * dx * du + dy * dv, sh1);
+                            sh2 = fma(tq, dx * (du * em - dv * el), sh2);
+                        }
+                    }
+                } else {
+                    // sersic: D = 1 + r2, f = D^-3/2, c = -3 f / D; d f / d(e1,
+                    // e2, R) = c (kR' (dx du - dy dv) + a1 r2, kR' (dx dv + dy du)
+                    // + a2 r2, k/den (dx dx0 + dy dy0)) with dx0 = du (1 + e1)
+                    // + dv e2, dy0 = du e2 + dv (1 - e1)
+                    const T kR = h[1], a1 = h[2], a2 = h[3], kd = h[4], e1 = h[5], e2 = h[6];
+#pragma unroll 1
+                    for (int q = 0; q < NA; ++q) {
+                        const J4<T> Y = pair_y(q);
+                        const T dx = Lx[q * st] - xp, dy = Ly[q * st] - yp;
+                        const T r2 = fma(dx, dx, dy * dy);
+                        T id;
+                        const T f = sersic_factor(r2, id);
+                        accumulate(f, Y);
+                        if (q > p) {
+                            const T du = Luv[q] - up, dv = Luv[NA + q] - vp;
+                            const T tq = T(-6) * f * id * jdot_re(Jp, Y);
+                            sh0 = fma(tq, kR * (dx * du - dy * dv) + a1 * r2, sh0);
+                            sh1 = fma(tq, kR * (dx * dv + dy * du) + a2 * r2, sh1);
+                            sh2 = fma(tq, kd * (dx * (du * (T(1) + e1) + dv * e2) + dy * (du * e2 + dv * (T(1) - e1))),
+                                      sh2);
+                        }
+                    }
                 }
             }
             __syncthreads();                       // every X read done
@@ -500,12 +586,16 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
                     const double *an = Lant + 4 * a;
                     const T *jb = Jbuf + (it & 1) * jplane + a * st + si;
                     const int js = NA * st;
-                    auto Jb = [&](int x) { return cx<T>{jb[(2 * x) * js], jb[(2 * x + 1) * js]}; };
+                    J4<T> Jb;
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) Jb.v[x] = {jb[(2 * x) * js], jb[(2 * x + 1) * js]};
                     double gr[7];
-                    item_adjoint<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2], an[3], c,
-                                    Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si], T(prm(it, 3)[si]),
-                                    T(prm(it, 4)[si]), T(prm(it, 5)[si]), T(prm(it, 6)[si]),
-                                    T(prm(it, 7)[si]), Jb, gr);
+                    ItemState<T> ist;
+                    (void)item_forward<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2],
+                                          an[3], c, Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si],
+                                          T(prm(it, 3)[si]), T(prm(it, 4)[si]), T(prm(it, 5)[si]),
+                                          T(prm(it, 6)[si]), T(prm(it, 7)[si]), ist);
+                    item_backward<T>(ist, Jb, gr);
                     double *sa = sacc + ((it / S.wt) & 1) * wsrc * 7 + ((it % S.wt) * st + si) * 7;
 #pragma unroll
                     for (int x = 0; x < 7; ++x) atomicAdd(sa + x, gr[x]);

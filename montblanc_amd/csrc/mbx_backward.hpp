@@ -218,142 +218,6 @@ __device__ __forceinline__ void item_backward(const ItemState<T> &st, const J4<T
     g[1] = double(g_phi) * st.dphi_dm + double(g_ld) * st.dvl_dm + double(g_md) * st.dvm_dm;
 }
 
-// Forward and backward of one item in one pass when its Jones adjoint is
-// already known (inputs as item_forward; jb(x) returns entry x of Jb, read
-// where it is needed rather than held): d chi^2 / d(l, m, I, Q, U, V,
-// alpha) into g. The phase and sqrt(B) come first, so that the beam can be
-// evaluated one correlation at a time and each correlation's value and
-// l, m derivatives used at once (Eb = Jb X^H is known by then, and
-// Xb = E^H Jb accumulates entry by entry): no beam state is held.
-template <typename T, typename JbAt>
-__device__ __forceinline__ void item_adjoint(const RimeArgs<T> &A, const double *xa, double ps,
-                                             double pc, double pel, double pem, double al,
-                                             double am, int c, double f, double lnr, double l0,
-                                             double m0, T I, T Q, T U, T V, T P, const JbAt &jb,
-                                             double *g /* [7] */) {
-    const mbx_dims &d = A.d;
-    const double n = sqrt(1.0 - l0 * l0 - m0 * m0), rn = 1.0 / n;
-    const double k2 = -A.two_pi_over_c * f;
-    const double dphi_dl = k2 * (xa[0] - xa[2] * l0 * rn), dphi_dm = k2 * (xa[1] - xa[2] * m0 * rn);
-    // ---- beam grid position (as item_forward) and its l, m derivatives
-    const double lr = (l0 * pc - m0 * ps + pel) * al;
-    const double mr = (l0 * ps + m0 * pc + pem) * am;
-    const double LW1 = double(d.beam_lw.local_size - 1), MH1 = double(d.beam_mh.local_size - 1);
-    const double vl_raw = A.lscale * (lr - A.ll), vm_raw = A.mscale * (mr - A.lmin);
-    const double vl = fmin(LW1, fmax(0.0, vl_raw)), vm = fmin(MH1, fmax(0.0, vm_raw));
-    const bool okl = vl_raw >= 0.0 && vl_raw <= LW1, okm = vm_raw >= 0.0 && vm_raw <= MH1;
-    const double dvl_dl = okl ? A.lscale * al * pc : 0.0, dvl_dm = okl ? -A.lscale * al * ps : 0.0;
-    const double dvm_dl = okm ? A.mscale * am * ps : 0.0, dvm_dm = okm ? A.mscale * am * pc : 0.0;
-    const double gl0 = floor(vl), gm0 = floor(vm);
-    const T ld = T(vl - gl0), md = T(vm - gm0);
-    const uint32_t MH = d.beam_mh.local_size;
-    const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
-    const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const T *plane = A.beam[uint64_t(c) * d.beam_lw.local_size * MH].v;
-    const uint32_t cell[4] = {(il0 * MH + im0) * 12, (il1 * MH + im0) * 12, (il0 * MH + im1) * 12,
-                              (il1 * MH + im1) * 12};
-    // ---- phase and sqrt(B) = sqrt(P) sqrt(B0)
-    T sn, cs;
-    PhaseCS<T>::eval(k2 * (xa[2] * (n - 1.0) + xa[1] * m0 + xa[0] * l0), &sn, &cs);
-    const cx<T> K = {cs, sn};
-    const T rP = sqrt(P);
-    const T s0 = sqrt(I * I - Q * Q - U * U - V * V);
-    const T t0 = sqrt(T(2) * I + T(2) * s0);
-    J4<T> B0s;
-    B0s.v[0] = {I + Q + s0, T(0)}; B0s.v[1] = {U, V}; B0s.v[2] = {U, -V}; B0s.v[3] = {I - Q + s0, T(0)};
-    const T it0 = t0 != T(0) ? T(1) / t0 : T(0);
-    if (t0 != T(0)) B0s = jscale(B0s, it0);
-    // Eb = Jb X^H, X = K sqrt(B)
-    J4<T> Eb;
-    {
-        J4<T> X;
-#pragma unroll
-        for (int x = 0; x < 4; ++x) X.v[x] = cmul(K, cx<T>{B0s.v[x].re * rP, B0s.v[x].im * rP});
-        J4<T> Jb;
-#pragma unroll
-        for (int x = 0; x < 4; ++x) Jb.v[x] = jb(x);
-        Eb = jmul_h(Jb, X);
-    }
-    T g_ld = T(0), g_md = T(0);
-    J4<T> Xb = jzero<T>();                          // E^H Jb
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        cx<T> S = {T(0), T(0)}, dSl = S, dSm = S;
-        T Am = T(0), dAl = T(0), dAm = T(0);
-#pragma unroll
-        for (int tap = 0; tap < 4; ++tap) {
-            const bool hl = tap & 1, hm = (tap >> 1) & 1;
-            const T wl = hl ? ld : T(1) - ld, wm = hm ? md : T(1) - md;
-            const T w = wl * wm, wdl = hl ? wm : -wm, wdm = hm ? wl : -wl;
-            const T *e = plane + cell[tap] + 3 * x;
-            const T vre = e[0], vim = e[1], vamp = e[2];
-            S.re = fma(w, vre, S.re);       S.im = fma(w, vim, S.im);
-            dSl.re = fma(wdl, vre, dSl.re); dSl.im = fma(wdl, vim, dSl.im);
-            dSm.re = fma(wdm, vre, dSm.re); dSm.im = fma(wdm, vim, dSm.im);
-            Am = fma(w, vamp, Am); dAl = fma(wdl, vamp, dAl); dAm = fma(wdm, vamp, dAm);
-        }
-        cx<T> E, dEl, dEm;
-        const T n2 = S.re * S.re + S.im * S.im;
-        if (n2 > T(0) && isfinite(n2)) {
-            const T inv = rsqrt_acc(n2);
-            const T gn = Am * inv;
-            E = {S.re * gn, S.im * gn};
-            const T rl = (S.re * dSl.re + S.im * dSl.im) * inv * inv;
-            const T rm = (S.re * dSm.re + S.im * dSm.im) * inv * inv;
-            const T cl = dAl * inv - gn * rl, cm = dAm * inv - gn * rm;
-            dEl = {dSl.re * gn + S.re * cl, dSl.im * gn + S.im * cl};
-            dEm = {dSm.re * gn + S.re * cm, dSm.im * gn + S.im * cm};
-        } else {
-            E = {S.re * Am, S.im * Am};
-            dEl = {dSl.re * Am + S.re * dAl, dSl.im * Am + S.im * dAl};
-            dEm = {dSm.re * Am + S.re * dAm, dSm.im * Am + S.im * dAm};
-        }
-        g_ld += Eb.v[x].re * dEl.re + Eb.v[x].im * dEl.im;
-        g_md += Eb.v[x].re * dEm.re + Eb.v[x].im * dEm.im;
-        // Xb[i][j] += conj(E[k][i]) Jb[k][j], x = 2 k + i
-        const int k = x >> 1, i = x & 1;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const cx<T> b = jb(2 * k + j);
-            Xb.v[2 * i + j].re += E.re * b.re + E.im * b.im;
-            Xb.v[2 * i + j].im += E.re * b.im - E.im * b.re;
-        }
-        // one correlation's taps in flight at a time: bounds the registers
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    // ---- phase, alpha and Stokes from Xb (as item_backward)
-    cx<T> Kb = {T(0), T(0)};
-    T ZsB = T(0);                                   // Re <Zb, sqrt(B)>
-    J4<T> Zb;
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        const T br = B0s.v[x].re * rP, bi = B0s.v[x].im * rP;
-        Kb.re += Xb.v[x].re * br + Xb.v[x].im * bi;
-        Kb.im += Xb.v[x].im * br - Xb.v[x].re * bi;
-        Zb.v[x] = cmul(cx<T>{K.re, -K.im}, Xb.v[x]);
-        ZsB += Zb.v[x].re * br + Zb.v[x].im * bi;
-    }
-    const T g_phi = T(-2) * (K.im * Kb.re - K.re * Kb.im);
-    g[6] = double(lnr) * double(ZsB);
-    if (t0 != T(0) && s0 != T(0)) {
-        const T ZI = Zb.v[0].re + Zb.v[3].re;
-        const T ZB = jdot_re(Zb, B0s);
-        const T rdB[4] = {Zb.v[0].re + Zb.v[3].re, Zb.v[0].re - Zb.v[3].re,
-                          Zb.v[1].re + Zb.v[2].re, Zb.v[1].im - Zb.v[2].im};
-        const T is0 = T(1) / s0;
-        const T dsx[4] = {I * is0, -Q * is0, -U * is0, -V * is0};
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            const T dt = (x == 0 ? T(1) : T(0)) + dsx[x];
-            g[2 + x] = 2.0 * double(rP) * double((rdB[x] + dsx[x] * ZI) * it0 - ZB * dt * it0 * it0);
-        }
-    } else {
-        g[2] = g[3] = g[4] = g[5] = 0.0;
-    }
-    g[0] = double(g_phi) * dphi_dl + double(T(2) * g_ld) * dvl_dl + double(T(2) * g_md) * dvm_dl;
-    g[1] = double(g_phi) * dphi_dm + double(T(2) * g_ld) * dvl_dm + double(T(2) * g_md) * dvm_dm;
-}
-
 // Forward and backward of one item in one call (inputs as item_forward).
 template <typename T>
 __device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const double *xa,
