@@ -297,6 +297,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             // q > p only and counted twice.
             const bool mine = tid < nst * NA;
             const int si = mine ? tid % nst : 0, p = mine ? tid / nst : 0;
+            // two pairs in flight in float (registers to spare), one in float64
+            constexpr int kQUnroll = sizeof(T) == 4 ? 2 : 1;
             J4<T> acc = jzero<T>();
             T sh0 = T(0), sh1 = T(0), sh2 = T(0);
             if (mine) {
@@ -331,7 +333,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     // gaussian: f = exp(-r2); d f / d(el, em, R) = -2 f k
                     // (dy du - R dx dv, R dx du + dy dv, dx (du em - dv el))
                     const T k = h[1], el = h[3], em = h[4], Rg = h[5];
-#pragma unroll 1
+#pragma unroll kQUnroll
                     for (int q = 0; q < NA; ++q) {
                         const J4<T> Y = pair_y(q);
                         const T dx = Lx[q * st] - xp, dy = Ly[q * st] - yp;
@@ -351,7 +353,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     // + a2 r2, k/den (dx dx0 + dy dy0)) with dx0 = du (1 + e1)
                     // + dv e2, dy0 = du e2 + dv (1 - e1)
                     const T kR = h[1], a1 = h[2], a2 = h[3], kd = h[4], e1 = h[5], e2 = h[6];
-#pragma unroll 1
+#pragma unroll kQUnroll
                     for (int q = 0; q < NA; ++q) {
                         const J4<T> Y = pair_y(q);
                         const T dx = Lx[q * st] - xp, dy = Ly[q * st] - yp;
