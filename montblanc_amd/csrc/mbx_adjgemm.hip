@@ -74,9 +74,18 @@ static __device__ __forceinline__ double sersic_factor(double r2, double &id) {
     return id * sqrt(id);
 }
 
-template <typename T, bool SHAPED>
-__global__ void __launch_bounds__(kAdjThreads)
+// WIDE (shaped pass only): 1024 threads. The items' forward state is not
+// held across the pair loop but recomputed for the backward, which leaves
+// registers for sixteen waves; each (source, antenna p) item's pair loop is
+// split over two threads (q in either half of the antennas), whose partial
+// Jones adjoints meet in LDS.
+template <bool WIDE> constexpr int adjg_threads() { return WIDE ? 1024 : kAdjThreads; }
+
+template <typename T, bool SHAPED, bool WIDE = false>
+__global__ void __launch_bounds__((adjg_threads<WIDE>()))
 k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
+    static_assert(SHAPED || !WIDE, "the wide layout is for the shaped pass");
+    constexpr int kThreads = adjg_threads<WIDE>();
     extern __shared__ __align__(32) unsigned char smem_raw[];
     const RimeArgs<T> &R = A.R;
     const mbx_dims &d = R.d;
@@ -119,15 +128,15 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     T *Lxy = Lst + st * 16;
     T *Luv = Lxy + 2 * NA * st;
 
-    tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kAdjThreads);
-    stage_antennas<T>(R, Lant, t, c, tid, kAdjThreads);
-    for (int i = tid; i < 2 * rows * rows; i += kAdjThreads) Wr[i] = T(0);
-    for (int i = tid; i < S.sw * 10; i += kAdjThreads) sacc[i] = 0.0;
+    tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kThreads);
+    stage_antennas<T>(R, Lant, t, c, tid, kThreads);
+    for (int i = tid; i < 2 * rows * rows; i += kThreads) Wr[i] = T(0);
+    for (int i = tid; i < S.sw * 10; i += kThreads) sacc[i] = 0.0;
     __syncthreads();
     if (SHAPED)
-        for (int a = tid; a < NA; a += kAdjThreads) { Luv[a] = T(Luvw[3 * a]); Luv[NA + a] = T(Luvw[3 * a + 1]); }
+        for (int a = tid; a < NA; a += kThreads) { Luv[a] = T(Luvw[3 * a]); Luv[NA + a] = T(Luvw[3 * a + 1]); }
     // W from this (t, c)'s visibility adjoints
-    for (int b = tid; b < nb_ext; b += kAdjThreads) {
+    for (int b = tid; b < nb_ext; b += kThreads) {
         const int p = min(max(R.a1[uint64_t(t) * NBL + b], 0), NA - 1);
         const int q = min(max(R.a2[uint64_t(t) * NBL + b], 0), NA - 1);
         const J4<T> V = jload(A.vbar + ((uint64_t(t) * NBL + b) * NC + c) * 4);
@@ -163,7 +172,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         const int nst = min(st, send - s0);
         double *sac = sacc + (s0 - wbase) * 10;    // this tile's sums
         // ---- the tile's source parameters, once
-        for (int si = tid; si < nst; si += kAdjThreads) {
+        for (int si = tid; si < nst; si += kThreads) {
             double l0, m0, nm1;
             const J4<T> B = source_sqrtb<T>(R, Lch, t, s0 + si, l0, m0, nm1);
             Lsd[4 * si] = l0; Lsd[4 * si + 1] = m0; Lsd[4 * si + 2] = nm1;
@@ -205,9 +214,14 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         if (own) {
             const double *an = Lant + 4 * ia;
             const T *o = Lst + 16 * isi;
-            const J4<T> J = item_forward<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[0], an[1],
-                                            an[2], an[3], c, Lch[3], Lch[4], Lsd[4 * isi],
-                                            Lsd[4 * isi + 1], o[0], o[1], o[2], o[3], o[4], ist);
+            J4<T> J;
+            if (WIDE)
+                J = staged_jones<T, 4>(R, Luvw, Lpa, Lant, Lch, c, ia, Lsd[4 * isi], Lsd[4 * isi + 1],
+                                       Lsd[4 * isi + 2], reinterpret_cast<const J4<T> *>(o + 8)[0]);
+            else
+                J = item_forward<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[0], an[1], an[2],
+                                    an[3], c, Lch[3], Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], o[0], o[1],
+                                    o[2], o[3], o[4], ist);
 #pragma unroll
             for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -233,7 +247,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 Lxy[(NA + ia) * st + isi] = T(y);
             }
         }
-        for (int i = tid; i < 2 * st * rows; i += kAdjThreads) {
+        for (int i = tid; i < (SHAPED ? 0 : 2 * st * rows); i += kThreads) {
             const int n = SHAPED ? i % (2 * st) : (i >> 2) % (2 * st);
             const int r = SHAPED ? i / (2 * st) : (i / (8 * st)) * 4 + (i & 3);
             if (n >= 2 * nst || r >= 2 * NA) { Xr[i] = T(0); Xi[i] = T(0); }
@@ -295,8 +309,12 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             // (f and its derivatives are even in the pair's (u, v), and
             // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): it is formed for
             // q > p only and counted twice.
-            const bool mine = tid < nst * NA;
-            const int si = mine ? tid % nst : 0, p = mine ? tid / nst : 0;
+            const int half = WIDE && tid >= nst * NA ? 1 : 0;
+            const int r = tid - half * nst * NA;
+            const bool mine = r < nst * NA;
+            const int si = mine ? r % nst : 0, p = mine ? r / nst : 0;
+            // pair range: all antennas, or one half of them (WIDE)
+            const int q0 = WIDE && half ? NA / 2 : 0, q1 = WIDE && !half ? NA / 2 : NA;
             // two pairs in flight in float (registers to spare), one in float64
             constexpr int kQUnroll = sizeof(T) == 4 ? 2 : 1;
             J4<T> acc = jzero<T>();
@@ -311,16 +329,28 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         Jp.v[2 * k + j] = {Xr[xi_(2 * si + j, 2 * p + k)], Xi[xi_(2 * si + j, 2 * p + k)]};
                 const T *Lx = Lxy + si, *Ly = Lxy + NA * st + si;
                 const T xp = Lx[p * st], yp = Ly[p * st], up = Luv[p], vp = Luv[NA + p];
+                // running LDS offsets of pair q: W rows 2p, 2p + 1 at columns
+                // 2q, 2q + 1 (wo + 4 i + k), X rows 2q, 2q + 1 of source si
+                // (xo + 2 st i + k), shape coordinates (lo)
+                uint32_t wo = (uint32_t(q0 >> 1) * rows + 2 * p) * 4 + 2 * (q0 & 1);
+                uint32_t xo = uint32_t(2 * q0) * (2 * st) + 2 * si;
+                uint32_t lo = uint32_t(q0) * st;
+                const uint32_t xs2 = 2 * st;
                 auto pair_y = [&](int q) {
                     J4<T> Wq, Jq;
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int k = 0; k < 2; ++k) {
-                            Wq.v[2 * i + k] = {Wr[wi_(2 * p + i, 2 * q + k)], Wi[wi_(2 * p + i, 2 * q + k)]};
-                            Jq.v[2 * i + k] = {Xr[xi_(2 * si + k, 2 * q + i)], Xi[xi_(2 * si + k, 2 * q + i)]};
+                            Wq.v[2 * i + k] = {Wr[wo + 4 * i + k], Wi[wo + 4 * i + k]};
+                            Jq.v[2 * i + k] = {Xr[xo + xs2 * i + k], Xi[xo + xs2 * i + k]};
                         }
                     return jmul(Wq, Jq);
+                };
+                auto advance = [&](int q) {
+                    wo += (q & 1) ? 4 * uint32_t(rows) - 2 : 2;
+                    xo += 2 * xs2;
+                    lo += st;
                 };
                 auto accumulate = [&](T f, const J4<T> &Y) {
 #pragma unroll
@@ -334,9 +364,10 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     // (dy du - R dx dv, R dx du + dy dv, dx (du em - dv el))
                     const T k = h[1], el = h[3], em = h[4], Rg = h[5];
 #pragma unroll kQUnroll
-                    for (int q = 0; q < NA; ++q) {
+                    for (int q = q0; q < q1; ++q) {
                         const J4<T> Y = pair_y(q);
-                        const T dx = Lx[q * st] - xp, dy = Ly[q * st] - yp;
+                        const T dx = Lx[lo] - xp, dy = Ly[lo] - yp;
+                        advance(q);
                         const T f = gauss_factor(fma(dx, dx, dy * dy));
                         accumulate(f, Y);
                         if (q > p) {
@@ -354,9 +385,10 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     // + dv e2, dy0 = du e2 + dv (1 - e1)
                     const T kR = h[1], a1 = h[2], a2 = h[3], kd = h[4], e1 = h[5], e2 = h[6];
 #pragma unroll kQUnroll
-                    for (int q = 0; q < NA; ++q) {
+                    for (int q = q0; q < q1; ++q) {
                         const J4<T> Y = pair_y(q);
-                        const T dx = Lx[q * st] - xp, dy = Ly[q * st] - yp;
+                        const T dx = Lx[lo] - xp, dy = Ly[lo] - yp;
+                        advance(q);
                         const T r2 = fma(dx, dx, dy * dy);
                         T id;
                         const T f = sersic_factor(r2, id);
@@ -374,25 +406,48 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             }
             __syncthreads();                       // every X read done
             if (mine) {
-                Jb[si * NA + p] = acc;
+                if (!WIDE || half) Jb[si * NA + p] = acc;
                 atomicAdd(sac + si * 10 + 7, double(sh0));
                 atomicAdd(sac + si * 10 + 8, double(sh1));
                 atomicAdd(sac + si * 10 + 9, double(sh2));
             }
+            if (WIDE) {
+                // the first half's threads own the items: add the second
+                // half's partial and run the backward with the forward state
+                // recomputed
+                __syncthreads();
+                if (mine && !half) {
+                    const J4<T> Jo = Jb[si * NA + p];
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) {
+                        acc.v[x].re += Jo.v[x].re;
+                        acc.v[x].im += Jo.v[x].im;
+                    }
+                    const double *an = Lant + 4 * p;
+                    const T *o = Lst + 16 * si;
+                    (void)item_forward<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[0], an[1], an[2],
+                                          an[3], c, Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1], o[0], o[1],
+                                          o[2], o[3], o[4], ist);
+                    double g[7];
+                    item_backward<T>(ist, acc, g);
+#pragma unroll
+                    for (int x = 0; x < 7; ++x) atomicAdd(sac + si * 10 + x, g[x]);
+                }
+            }
         }
         __syncthreads();
         // ---- backward of J = E K sqrt(B) per (source, antenna)
-        if (own) {
+        if (own && !WIDE) {
             double g[7];
             item_backward<T>(ist, Jb[isi * NA + ia], g);
 #pragma unroll
             for (int x = 0; x < 7; ++x) atomicAdd(sac + isi * 10 + x, g[x]);
         }
-        __syncthreads();
+        if (!WIDE) __syncthreads();
         if (s0 + st < send && s0 + 2 * st <= wbase + S.sw) continue;   // window not full
         // ---- this workgroup's share of the window's gradients
         const int nwin = s0 + nst - wbase;
-        for (int si = tid; si < nwin; si += kAdjThreads) {
+        for (int si = tid; si < nwin; si += kThreads) {
             const int s = wbase + si;
             double *v = sacc + si * 10;
             const double gs = A.gscale;
@@ -763,7 +818,13 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
         if (r) return r;
     }
     if (dext(d.nsrc) > dext(d.npsrc)) {
-        hipLaunchKernelGGL((k_adj_gemm<T, true>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
+        // float: sixteen waves with the forward state recomputed; float64
+        // keeps it in registers (eight waves)
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((k_adj_gemm<T, true, true>), dim3(nwg), dim3(adjg_threads<true>()), lds, stream,
+                               Ag, S);
+        else
+            hipLaunchKernelGGL((k_adj_gemm<T, true>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
         return hip_status(hipGetLastError(), "adjoint shaped launch");
     }
     return MBX_OK;
