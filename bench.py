@@ -133,9 +133,9 @@ def time_forward(w, dtype, steps, warmup):
                 tflops=flops / (ms_mean * 1e-3) / 1e12)
 
 
-def time_gradient(w, dtype, steps, warmup):
+def time_gradient(w, dtype, steps, warmup, name='gradient_27ant'):
     """chi^2 alone vs chi^2 + full gradient (forward keeping the pre-gain
-    sums, then the adjoint) on the gradient workload."""
+    sums, then the adjoint) on workload w."""
     import numpy as np
     import torch
     from montblanc_amd import rime
@@ -175,7 +175,7 @@ def time_gradient(w, dtype, steps, warmup):
     n_terms = sum(counts) * w['ntime'] * nbl * w['nchan']
     del arrays, out, grads
     torch.cuda.empty_cache()
-    return {'workload': 'gradient_27ant', 'dtype': dtype, 'terms': n_terms,
+    return {'workload': name, 'dtype': dtype, 'terms': n_terms,
             'forward_ms': res['forward'], 'forward_gradient_ms': res['forward_gradient'],
             'forward_terms_per_s': n_terms / (res['forward'] * 1e-3),
             'gradient_terms_per_s': n_terms / (res['forward_gradient'] * 1e-3),
@@ -211,6 +211,11 @@ def time_large(steps=1):
     mod, _, need = planner.plan(dims, np.float32, budget - planner.DEVICE_RESERVE, gradients=True)
     out = rime.alloc_outputs(arrays, keep_ekb=True)
     grads = rime.alloc_grads(arrays, counts)
+    # warm-up on one time step: launch plans, baseline map, workspaces
+    probe = rime.make_dims(arrays, counts)
+    probe.ntime.upper_extent = 1
+    rime.forward(arrays, counts, outputs=out, dims=probe)
+    rime.adjoint(arrays, counts, out['ekb_vis'], grads=grads, dims=probe)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -387,6 +392,12 @@ def main():
               % (grad['forward_ms'], grad['forward_gradient_ms'], grad['gradient_over_forward']),
               file=sys.stderr, flush=True)
         line['gradient'] = grad
+        # float32 (float64 phase) on the forward workload itself
+        g32 = time_gradient(wl, 'f32', max(3, args.steps // 2), 1, name=args.workload)
+        print('# gradient f32 (%s): forward %.2f ms, forward+gradient %.2f ms, ratio %.3f'
+              % (args.workload, g32['forward_ms'], g32['forward_gradient_ms'], g32['gradient_over_forward']),
+              file=sys.stderr, flush=True)
+        line['gradient_f32'] = g32
     if not args.no_batched:
         line['batched'] = time_batched()
     if args.large:
