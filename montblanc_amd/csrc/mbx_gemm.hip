@@ -114,9 +114,8 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     double *Lpa = Luvw + 3 * NA;
     double *Lant = Lpa + 2 * NA;
     double *Lch = Lant + 4 * NA;
-    // two source-tile buffers: Jones (point tiles X[s][row], shaped tiles
-    // component-major Xg[k][row][s]), shape coordinates XY[2][a][s] and
-    // the source kinds of shaped tiles
+    // two source-tile buffers: Jones X[s][row] (four reals per row), the
+    // shaped tiles' shape coordinates XY[s][a] = (x, y) and source kinds
     T *Xbase = reinterpret_cast<T *>(smem_raw + gemm_x_offset(NA));
     const int xstride = 4 * st * rows;       // T per Jones buffer
     T *XYbase = Xbase + 2 * xstride;
@@ -194,9 +193,12 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
                 }
             } else {
                 // one source per MFMA (k = the four reals of its Jones
-                // row), each product scaled per element by f_pq
+                // row), each product scaled per element by f_pq. Same
+                // source-major rows as the point pass, so a lane's read of
+                // one real of a row is conflict-free; shape coordinates
+                // (x, y) interleaved per (source, antenna).
                 const T *Xg = Xbase + buf * xstride;
-                const T *Xc = XYbase + buf * xystride, *Yc = Xc + NA * st;
+                const T *XY = XYbase + buf * xystride;
                 const int *kind = Kbase + buf * st;
                 for (int s = 0; s < nst; ++s) {
                     // re-derived per source so the per-tile-pair addresses
@@ -207,22 +209,23 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
                     const int g2 = (gv + 2) & 3;
                     const T sg = gv < 2 ? T(1) : T(-1);
                     const bool gauss = kind[s] == 1;
+                    const T *Xs = Xg + s * rows * 4;
+                    const T *XYs = XY + s * NA * 2;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
                             const int ra = 16 * tpR[j] + Mfma<T>::arow(i16v), cb = 16 * tpC[j] + i16v;
-                            const acc_t P = Mfma<T>::mma(Xg[(gv * rows + ra) * st + s],
-                                                         Xg[(gv * rows + cb) * st + s], zero);
-                            const acc_t Q = Mfma<T>::mma(sg * Xg[(g2 * rows + ra) * st + s],
-                                                         Xg[(gv * rows + cb) * st + s], zero);
+                            const T bv = Xs[cb * 4 + gv];
+                            const acc_t P = Mfma<T>::mma(Xs[ra * 4 + gv], bv, zero);
+                            const acc_t Q = Mfma<T>::mma(sg * Xs[ra * 4 + g2], bv, zero);
                             const int q = min(cb >> 1, NA - 1);
-                            const T xq = Xc[q * st + s], yq = Yc[q * st + s];
+                            const T xq = XYs[2 * q], yq = XYs[2 * q + 1];
                             // this lane's rows are antennas p0 = 8 R + 2 g and p0 + 1
                             // (rows past 2 na: any finite factor, discarded)
                             const int p0 = 8 * tpR[j] + 2 * gv;
                             const int pa0 = min(p0, NA - 1), pa1 = min(p0 + 1, NA - 1);
-                            const T f0 = shape_f<T>(gauss, xq - Xc[pa0 * st + s], yq - Yc[pa0 * st + s]);
-                            const T f1 = shape_f<T>(gauss, xq - Xc[pa1 * st + s], yq - Yc[pa1 * st + s]);
+                            const T f0 = shape_f<T>(gauss, xq - XYs[2 * pa0], yq - XYs[2 * pa0 + 1]);
+                            const T f1 = shape_f<T>(gauss, xq - XYs[2 * pa1], yq - XYs[2 * pa1 + 1]);
                             are[j][0] = fma(f0, P[0], are[j][0]); aim[j][0] = fma(f0, Q[0], aim[j][0]);
                             are[j][1] = fma(f0, P[1], are[j][1]); aim[j][1] = fma(f0, Q[1], aim[j][1]);
                             are[j][2] = fma(f1, P[2], are[j][2]); aim[j][2] = fma(f1, Q[2], aim[j][2]);
@@ -292,14 +295,14 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
                 for (int i = ptid; i < (nst4 - nst) * rows; i += pth)
                     Xb[(nst + i / rows) * rows + i % rows] = XRow<T>{{T(0), T(0), T(0), T(0)}};
             } else {
-                // Xg[k][row][s], XY[2][a][s]: consecutive threads take
-                // consecutive sources (conflict-free scalar writes)
-                T *Xg = Xbase + buf * xstride;
-                T *Xc = XYbase + buf * xystride, *Yc = Xc + NA * st;
+                // X[s][row] as the point pass, XY[s][a] = (x, y): a group of
+                // consecutive threads shares a source and walks the antennas
+                XRow<T> *Xb = reinterpret_cast<XRow<T> *>(Xbase + buf * xstride);
+                T *XY = XYbase + buf * xystride;
                 int *kind = Kbase + buf * st;
-                const int astep = pth / nst;
-                const int si = ptid % nst, a0 = ptid / nst;
-                if (a0 < astep) {
+                const int gs = pth / nst;
+                const int si = ptid / gs, a0 = ptid % gs;
+                if (si < nst) {
                     const int s = s0 + si;
                     double l0, m0, nm1;
                     const J4<T> B = source_b(s, l0, m0, nm1);
@@ -319,28 +322,15 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
                         cu = k * (1.0 + e1); cv = k * e2; du = k * e2; dv = k * (1.0 - e1);
                     }
                     if (a0 == 0) kind[si] = gauss ? 1 : 2;
-                    for (int a = a0; a < NA; a += astep) {
+                    for (int a = a0; a < NA; a += gs) {
                         const J4<T> J = jones(a, l0, m0, nm1, B);
-                        const T jr[2][4] = {{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im},
-                                            {J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
-#pragma unroll
-                        for (int i = 0; i < 2; ++i)
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) Xg[(k * rows + 2 * a + i) * st + si] = jr[i][k];
+                        Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
+                        Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
                         const double u = Luvw[3 * a], v = Luvw[3 * a + 1];
-                        Xc[a * st + si] = T(cu * u + cv * v);
-                        Yc[a * st + si] = T(du * u + dv * v);
+                        XY[(si * NA + a) * 2] = T(cu * u + cv * v);
+                        XY[(si * NA + a) * 2 + 1] = T(du * u + dv * v);
                     }
                 }
-                // padding sources: zero Jones (no contribution) and zero
-                // coordinates (a finite factor)
-                const int pad = nst4 - nst;
-                for (int i = ptid; i < pad * (4 * rows + 2 * NA); i += pth) {
-                    const int sp = nst + i % pad, r = i / pad;
-                    if (r < 4 * rows) Xg[r * st + sp] = T(0);
-                    else Xc[(r - 4 * rows) * st + sp] = T(0);   // Xc, then Yc rows
-                }
-                for (int i = ptid; i < pad; i += pth) kind[nst + i] = 1;
             }
         };
         if (ntiles > 0) produce(0, 0);
