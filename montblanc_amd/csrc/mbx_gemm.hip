@@ -352,10 +352,10 @@ template <typename T>
 bool shaped_gemm_pays(const mbx_dims &d) {
     // Measured on MI355X: the per-element shape factors keep the shaped
     // pass VALU-bound; it beats the vector-unit kernel only in float64
-    // with many antennas (64 antennas, 600 gaussian + sersic sources:
-    // whole f64 forward 72 ms against 89 ms). In float32 the vector-unit
-    // kernel stays ahead (64 antennas: 28.6 ms against 31.5 ms), as it
-    // does at 27 antennas.
+    // with many antennas (measured at 64 antennas with 600 gaussian +
+    // sersic sources). In float32 the vector-unit kernel stays ahead (64
+    // antennas: whole forward 28.6 ms against 31.5 ms), as it does at 27
+    // antennas.
     return sizeof(T) == 8 && d.na.local_size >= 48;
 }
 
