@@ -49,8 +49,9 @@ static __host__ __device__ inline size_t adjg_setup_bytes(int na) {
 }
 
 template <typename T>
-static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw) {
+static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, int split = 1) {
     return adjg_setup_bytes(na) + 2 * size_t(rows) * rows * sizeof(T)   // W planes
+           + size_t(split - 1) * st * na * sizeof(J4<T>) + 16            // partial Jb (WIDE)
            + 2 * size_t(2 * st) * rows * sizeof(T)                      // X planes (then Jb)
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
            + size_t(st) * 4 * sizeof(double)                            // l, m, n - 1
@@ -74,18 +75,24 @@ static __device__ __forceinline__ double sersic_factor(double r2, double &id) {
     return id * sqrt(id);
 }
 
-// WIDE (shaped pass only): 1024 threads. The items' forward state is not
-// held across the pair loop but recomputed for the backward, which leaves
-// registers for sixteen waves; each (source, antenna p) item's pair loop is
-// split over two threads (q in either half of the antennas), whose partial
+// WIDE (shaped pass only): the items' forward state is not held across the
+// pair loop but recomputed for the backward, which leaves registers for
+// more waves; each (source, antenna p) item's pair loop is split over
+// several threads (q in consecutive ranges of the antennas), whose partial
 // Jones adjoints meet in LDS.
-template <bool WIDE> constexpr int adjg_threads() { return WIDE ? 1024 : kAdjThreads; }
+// Float only: 1024 threads, each item's pair loop split over two. (In
+// float64 the recomputed backward needs more registers than sixteen, or
+// twelve, waves leave; measured slower than keeping the state.)
+template <typename T, bool WIDE> constexpr int adjg_threads() { return WIDE ? 1024 : kAdjThreads; }
+template <typename T, bool WIDE> constexpr int adjg_split() { return WIDE ? 2 : 1; }
 
 template <typename T, bool SHAPED, bool WIDE = false>
-__global__ void __launch_bounds__((adjg_threads<WIDE>()))
+__global__ void __launch_bounds__((adjg_threads<T, WIDE>()))
 k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     static_assert(SHAPED || !WIDE, "the wide layout is for the shaped pass");
-    constexpr int kThreads = adjg_threads<WIDE>();
+    static_assert(sizeof(T) == 4 || !WIDE, "the wide layout is for float");
+    constexpr int kThreads = adjg_threads<T, WIDE>();
+    constexpr int kSplit = adjg_split<T, WIDE>();
     extern __shared__ __align__(32) unsigned char smem_raw[];
     const RimeArgs<T> &R = A.R;
     const mbx_dims &d = R.d;
@@ -127,6 +134,9 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     // and the antennas' u, v [2][na]
     T *Lxy = Lst + st * 16;
     T *Luv = Lxy + 2 * NA * st;
+    // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na]
+    J4<T> *Jpart = reinterpret_cast<J4<T> *>(
+        smem_raw + ((reinterpret_cast<unsigned char *>(Luv + 2 * NA) - smem_raw + 15) & ~size_t(15)));
 
     tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kThreads);
     stage_antennas<T>(R, Lant, t, c, tid, kThreads);
@@ -309,12 +319,12 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             // (f and its derivatives are even in the pair's (u, v), and
             // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): it is formed for
             // q > p only and counted twice.
-            const int half = WIDE && tid >= nst * NA ? 1 : 0;
-            const int r = tid - half * nst * NA;
+            const int part = min(tid / (nst * NA), kSplit - 1);
+            const int r = tid - part * nst * NA;
             const bool mine = r < nst * NA;
             const int si = mine ? r % nst : 0, p = mine ? r / nst : 0;
-            // pair range: all antennas, or one half of them (WIDE)
-            const int q0 = WIDE && half ? NA / 2 : 0, q1 = WIDE && !half ? NA / 2 : NA;
+            // pair range: all antennas, or the part-th of kSplit ranges (WIDE)
+            const int q0 = part * NA / kSplit, q1 = (part + 1) * NA / kSplit;
             // two pairs in flight in float (registers to spare), one in float64
             constexpr int kQUnroll = sizeof(T) == 4 ? 2 : 1;
             J4<T> acc = jzero<T>();
@@ -406,22 +416,26 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             }
             __syncthreads();                       // every X read done
             if (mine) {
-                if (!WIDE || half) Jb[si * NA + p] = acc;
+                if (!WIDE) Jb[si * NA + p] = acc;
+                else if (part) Jpart[((part - 1) * nst + si) * NA + p] = acc;
                 atomicAdd(sac + si * 10 + 7, double(sh0));
                 atomicAdd(sac + si * 10 + 8, double(sh1));
                 atomicAdd(sac + si * 10 + 9, double(sh2));
             }
             if (WIDE) {
-                // the first half's threads own the items: add the second
-                // half's partial and run the backward with the forward state
-                // recomputed
+                // the first range's threads own the items: add the other
+                // ranges' partials and run the backward with the forward
+                // state recomputed
                 __syncthreads();
-                if (mine && !half) {
-                    const J4<T> Jo = Jb[si * NA + p];
+                if (mine && !part) {
 #pragma unroll
-                    for (int x = 0; x < 4; ++x) {
-                        acc.v[x].re += Jo.v[x].re;
-                        acc.v[x].im += Jo.v[x].im;
+                    for (int h = 1; h < kSplit; ++h) {
+                        const J4<T> Jo = Jpart[((h - 1) * nst + si) * NA + p];
+#pragma unroll
+                        for (int x = 0; x < 4; ++x) {
+                            acc.v[x].re += Jo.v[x].re;
+                            acc.v[x].im += Jo.v[x].im;
+                        }
                     }
                     const double *an = Lant + 4 * p;
                     const T *o = Lst + 16 * si;
@@ -773,20 +787,21 @@ static bool adjp_shape(const mbx_dims &d, AdjPointShape &S, size_t &lds) {
 }
 
 template <typename T>
-static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds) {
+static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds, int threads = kAdjThreads,
+                       int split = 1) {
     const int na = d.na.local_size;
     if (na <= 0 || na > 64) return false;
     S.rows = ((2 * na + 15) / 16) * 16;
-    S.st = std::min(16, (kAdjThreads / na) / 8 * 8);
+    S.st = std::min(16, (threads / split / na) / 8 * 8);
     if (S.st < 8) return false;
     static size_t budget = gemm_lds_budget();
-    auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw) <= budget; };
+    auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw, split) <= budget; };
     if (!fits(S.st, S.st) && S.st > 8) S.st = 8;
     if (!fits(S.st, S.st)) return false;
     // the widest flush window that fits, up to 32 tiles
     S.sw = S.st;
     while (S.sw < 32 * S.st && fits(S.st, S.sw + S.st)) S.sw += S.st;
-    lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw);
+    lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw, split);
     return (S.rows / 16) * (S.st / 8) <= kAdjMfmaWaves * kAdjMaxTiles;
 }
 
@@ -818,12 +833,19 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
         if (r) return r;
     }
     if (dext(d.nsrc) > dext(d.npsrc)) {
-        // float: sixteen waves with the forward state recomputed; float64
-        // keeps it in registers (eight waves)
-        if constexpr (sizeof(T) == 4)
-            hipLaunchKernelGGL((k_adj_gemm<T, true, true>), dim3(nwg), dim3(adjg_threads<true>()), lds, stream,
-                               Ag, S);
-        else
+        // float: sixteen waves with the forward state recomputed (WIDE);
+        // float64, and float where the wide tiles do not fit: the state kept
+        // in registers (eight waves)
+        AdjGemmShape SW;
+        size_t wlds;
+        bool wide = false;
+        if constexpr (sizeof(T) == 4) {
+            constexpr int wth = adjg_threads<T, true>(), wsp = adjg_split<T, true>();
+            wide = adjg_shape<T>(d, SW, wlds, wth, wsp);
+            if (wide)
+                hipLaunchKernelGGL((k_adj_gemm<T, true, true>), dim3(nwg), dim3(wth), wlds, stream, Ag, SW);
+        }
+        if (!wide)
             hipLaunchKernelGGL((k_adj_gemm<T, true>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
         return hip_status(hipGetLastError(), "adjoint shaped launch");
     }
