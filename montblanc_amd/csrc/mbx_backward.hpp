@@ -41,12 +41,16 @@ __device__ __forceinline__ J4<T> jadd(const J4<T> &A, const J4<T> &B) {
     return r;
 }
 
-// Re <A, B>
+// Re <A, B> (one fused multiply-add chain)
 template <typename T>
 __device__ __forceinline__ T jdot_re(const J4<T> &A, const J4<T> &B) {
-    T s = T(0);
+    T s = A.v[0].re * B.v[0].re;
+    s = fma(A.v[0].im, B.v[0].im, s);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s += A.v[i].re * B.v[i].re + A.v[i].im * B.v[i].im;
+    for (int i = 1; i < 4; ++i) {
+        s = fma(A.v[i].re, B.v[i].re, s);
+        s = fma(A.v[i].im, B.v[i].im, s);
+    }
     return s;
 }
 

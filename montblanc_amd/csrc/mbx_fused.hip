@@ -299,19 +299,34 @@ __device__ __forceinline__ void pc_consume(const RimeArgs<T> &A, const PcTile<T>
             }
         } else {
             const T s0p = shape[4 * si], s1p = shape[4 * si + 1], s2p = shape[4 * si + 2];
+            // the source's shape coordinates, linear in the pair's (u, v):
+            // x = u c1 + v c2, y = u c3 + v c4 (gaussian: k = gauss_scale f,
+            // float pre-scaled by sqrt(log2 e) for the hardware exp2;
+            // sersic: k = 2 pi f / C times R')
+            T c1, c2, c3, c4;
+            if (ty == T(1)) {
+                const T kg = kf_g * kGaussExp2Scale<T>(), kR = kg * s2p;
+                c1 = s1p * kR; c2 = -s0p * kR; c3 = s0p * kg; c4 = s1p * kg;
+            } else {
+                const T kR = kf_s * s2p;
+                c1 = (T(1) + s0p) * kR; c2 = s1p * kR; c3 = c2; c4 = (T(1) - s0p) * kR;
+            }
             T f[PB][QB];
+            auto r2_of = [&](int i, int j) {
+                const T u = uq[j] - up[i], v = vq[j] - vp[i];
+                const T x = fma(u, c1, v * c2), y = fma(u, c3, v * c4);
+                return fma(x, x, y * y);
+            };
             if (ty == T(1)) {
 #pragma unroll
                 for (int i = 0; i < PB; ++i)
 #pragma unroll
-                    for (int j = 0; j < QB; ++j)
-                        f[i][j] = gauss_f(uq[j] - up[i], vq[j] - vp[i], s0p, s1p, s2p, kf_g);
+                    for (int j = 0; j < QB; ++j) f[i][j] = gauss_of_scaled(r2_of(i, j));
             } else {
 #pragma unroll
                 for (int i = 0; i < PB; ++i)
 #pragma unroll
-                    for (int j = 0; j < QB; ++j)
-                        f[i][j] = sersic_f(uq[j] - up[i], vq[j] - vp[i], s0p, s1p, s2p, kf_s);
+                    for (int j = 0; j < QB; ++j) f[i][j] = sersic_of(r2_of(i, j));
             }
 #pragma unroll
             for (int j = 0; j < QB; ++j) {

@@ -563,6 +563,13 @@ __device__ __forceinline__ double sersic_of(double r2) {
     return 1.0 / (den * sqrt(den));
 }
 
+// Gaussian factor from coordinates pre-scaled by kGaussExp2Scale: float
+// exp2(-r2) on the hardware (coordinates scaled by sqrt(log2 e)), float64
+// exp(-r2).
+template <typename T> constexpr T kGaussExp2Scale() { return sizeof(T) == 4 ? T(1.2011224087864498) : T(1); }
+__device__ __forceinline__ float gauss_of_scaled(float r2) { return __builtin_amdgcn_exp2f(-r2); }
+__device__ __forceinline__ double gauss_of_scaled(double r2) { return exp(-r2); }
+
 template <typename T>
 __device__ __forceinline__ T gauss_f(T u, T v, T el, T em, T R, T k) {
     T x = (u * em - v * el) * k * R;
