@@ -100,6 +100,24 @@ def test_adjoint_many_point_tiles(dtype, na, counts):
         assert e < tol, (name, e)
 
 
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_adjoint_many_shaped_tiles(dtype):
+    """More gaussian and sersic sources than one gradient window holds (up
+    to 32 source tiles): the shaped pass's window flushes, and in float its
+    sixteen-wave layout with the pair loop split over two threads."""
+    counts = (4, 600, 30)
+    a = problem(27, 1, 2, counts, seed=77)
+    ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts, sigma_sqrd=1.3)
+    got, _ = gpu_grads(a, counts, dtype, False)
+    tol = TOL[dtype] if dtype == np.float64 else 3e-4
+    for name in NAMES:
+        if ref[name] is None or ref[name].size == 0:
+            continue
+        e = rel(got[name], ref[name])
+        print(name, e)
+        assert e < tol, (name, e)
+
+
 def test_autograd_function_and_gpu_finite_differences():
     """The torch.autograd.Function path gives the same gradients as the
     adjoint, and they match central differences (h = 1e-7 relative) of the
