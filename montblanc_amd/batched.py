@@ -99,11 +99,14 @@ class Batch(object):
 class _PerSample(object):
     """The per-sample strategy: every sample's chi^2 (and gradients) by the
     single-model entry points -- point sources on the matrix cores, the
-    adjoint's product too -- one after another on one stream, the whole
-    sequence captured as one HIP graph and replayed. The alternative to the
-    persistent launch when samples are large enough to fill the device.
+    adjoint's product too -- the whole sequence captured as one HIP graph
+    and replayed. The samples are spread over a few streams (each with its
+    own scratch outputs and workspaces, balanced by source count), so that
+    small samples run side by side instead of one small grid at a time.
 
     ntime_probe limits the time extent (autotuner probes)."""
+
+    STREAMS = 4
 
     def __init__(self, batch, gradients, use_weight_vector, sigma_sqrd, ntime_probe=None):
         from . import graph
@@ -124,24 +127,39 @@ class _PerSample(object):
         nchan = a['frequency'].shape[0]
         dev = batch.device
         ct = rime.complex_dtype(a)
-        self.out = dict(model_vis=torch.empty((ntime, nbl, nchan, 4), dtype=ct, device=dev),
-                        chi_sqrd_result=torch.empty((ntime, nbl, nchan), dtype=ft, device=dev))
+        ngroups = max(1, min(self.STREAMS, len(batch.counts)))
+        self.streams = [torch.cuda.Stream(dev) for _ in range(ngroups)]
+        self.workspaces = [rime.Workspace(dev) for _ in range(ngroups)]
+        outs = []
+        for _ in range(ngroups):
+            o = dict(model_vis=torch.empty((ntime, nbl, nchan, 4), dtype=ct, device=dev),
+                     chi_sqrd_result=torch.empty((ntime, nbl, nchan), dtype=ft, device=dev))
+            if gradients:
+                o['ekb_vis'] = torch.empty((ntime, nbl, nchan, 4), dtype=ct, device=dev)
+            outs.append(o)
         if gradients:
-            self.out['ekb_vis'] = torch.empty((ntime, nbl, nchan, 4), dtype=ct, device=dev)
             g = batch.grads_buffers()
-        self.samples = []
-        g0 = s0 = 0
+        # largest samples first, each to the least loaded stream
+        order = sorted(range(len(batch.counts)), key=lambda k: -sum(batch.counts[k]))
+        load = [0] * ngroups
+        self.groups = [[] for _ in range(ngroups)]
+        g0s = np.concatenate([[0], np.cumsum([c[1] for c in batch.counts])])
+        s0s = np.concatenate([[0], np.cumsum([c[2] for c in batch.counts])])
         o = batch.src_offset
-        for k, c in enumerate(batch.counts):
+        for k in order:
+            c = batch.counts[k]
+            grp = int(np.argmin(load))
+            load[grp] += sum(c)
             sa = dict(shared)
             for n in SOURCE_ROWS:
                 sa[n] = a[n][o[k]:o[k + 1]]
+            g0, s0 = int(g0s[k]), int(s0s[k])
             sa['gauss_shape'] = a['gauss_shape'][:, g0:g0 + c[1]].contiguous()
             sa['sersic_shape'] = a['sersic_shape'][:, s0:s0 + c[2]].contiguous()
             dims = rime.make_dims(sa, c)
             if ntime_probe is not None:
                 dims.ntime.upper_extent = min(dims.ntime.local_size, ntime_probe)
-            out = dict(self.out)
+            out = dict(outs[grp])
             out['X2'] = batch.X2[k]
             entry = dict(arrays=sa, counts=c, dims=dims, out=out, g0=g0, s0=s0)
             if gradients:
@@ -149,26 +167,35 @@ class _PerSample(object):
                                       alpha=g['alpha'][o[k]:o[k + 1]],
                                       gauss_shape=torch.empty((3, c[1]), dtype=torch.float64, device=dev),
                                       sersic_shape=torch.empty((3, c[2]), dtype=torch.float64, device=dev))
-            self.samples.append(entry)
-            g0 += c[1]
-            s0 += c[2]
+            self.groups[grp].append(entry)
         self.graph = graph.Graphed(self._run, device=dev)
 
-    def _run(self):
+    def _run_group(self, entries, ws):
         b = self.batch
-        for e in self.samples:
+        for e in entries:
             rime.forward(e['arrays'], e['counts'], outputs=e['out'], keep_ekb=self.gradients,
-                         dims=e['dims'], **self.kw)
+                         dims=e['dims'], workspace=ws, **self.kw)
             if self.gradients:
                 rime.adjoint(e['arrays'], e['counts'], e['out']['ekb_vis'], grads=e['grads'],
-                             dims=e['dims'], **self.kw)
+                             dims=e['dims'], workspace=ws, **self.kw)
                 g = b.grads_buffers()
                 c = e['counts']
                 if c[1]:
                     g['gauss_shape'][:, e['g0']:e['g0'] + c[1]].copy_(e['grads']['gauss_shape'])
                 if c[2]:
                     g['sersic_shape'][:, e['s0']:e['s0'] + c[2]].copy_(e['grads']['sersic_shape'])
-        return b.X2
+
+    def _run(self):
+        # fork onto the group streams and join back (also inside capture)
+        main = torch.cuda.current_stream(self.batch.device)
+        for s in self.streams:
+            s.wait_stream(main)
+        for s, entries, ws in zip(self.streams, self.groups, self.workspaces):
+            with torch.cuda.stream(s):
+                self._run_group(entries, ws)
+        for s in self.streams:
+            main.wait_stream(s)
+        return self.batch.X2
 
     def __call__(self):
         return self.graph.replay()

@@ -211,10 +211,12 @@ def pair_plan(arrays, pair_block=None, point_gemm=True):
 
 def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=None,
             outputs=None, keep_ekb=False, dims=None, stream=None, pair_block=None,
-            point_gemm=True):
+            point_gemm=True, workspace=None):
     """Fused forward on device tensors. Returns the outputs dict
     (model_vis, chi_sqrd_result, X2 [device float64 scalar], ekb_vis?).
-    point_gemm=False keeps the point sources on the vector units."""
+    point_gemm=False keeps the point sources on the vector units;
+    workspace (a Workspace) replaces the per-device one, for concurrent
+    calls on several streams."""
     dev = arrays['uvw'].device
     out = outputs if outputs is not None else alloc_outputs(arrays, keep_ekb)
     d = make_dims(arrays, counts) if dims is None else dims
@@ -224,7 +226,7 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
                             _raw(out.get('X2')), _raw(out.get('ekb_vis')))
     lib = _lib.load()
     nb = lib.mbx_forward_workspace_bytes(ctypes.byref(d), int(is_double(arrays)))
-    ws = _workspace(dev).get('forward', nb)
+    ws = (workspace or _workspace(dev)).get('forward', nb)
     b = beam_extents(arrays)
     k = consts(arrays, sigma_sqrd)
     h = stream_of(dev) if stream is None else ctypes.c_void_p(stream)
@@ -262,7 +264,7 @@ def alloc_grads(arrays, counts):
 
 
 def adjoint(arrays, counts, ekb_vis, grad_scale=1.0, use_weight_vector=False, sigma_sqrd=None,
-            grads=None, stream=None, dims=None):
+            grads=None, stream=None, dims=None, workspace=None):
     """d chi^2 / d(lm, stokes, alpha, gauss_shape, sersic_shape, G_term) times
     grad_scale, from the forward's pre-gain sums ``ekb_vis``. Returns a dict
     of float64 tensors (G_term complex128: d/dRe + i d/dIm)."""
@@ -274,7 +276,7 @@ def adjoint(arrays, counts, ekb_vis, grad_scale=1.0, use_weight_vector=False, si
     g = _lib.mbx_grads(**{n: _raw(grads.get(n)) for n in _lib.GRAD_NAMES})
     lib = _lib.load()
     nb = lib.mbx_adjoint_workspace_bytes(ctypes.byref(d), int(is_double(arrays)))
-    ws = _workspace(dev).get('adjoint', nb)
+    ws = (workspace or _workspace(dev)).get('adjoint', nb)
     b = beam_extents(arrays)
     k = consts(arrays, sigma_sqrd)
     h = stream_of(dev) if stream is None else ctypes.c_void_p(stream)
