@@ -263,8 +263,8 @@ def time_batched(steps=2):
     every sample, then chi^2 + gradient of every sample. The autotuner
     (run in the untimed warm-up) picks either one persistent launch over
     all samples with tuned tile shapes, or the per-sample strategy (each
-    sample through the single-model matrix-core kernels, the sequence
-    replayed as one HIP graph); the choice is reported."""
+    sample through the single-model matrix-core kernels on four streams,
+    the sequence replayed as one HIP graph); the choice is reported."""
     import numpy as np
     import torch
     from montblanc_amd import autotune, batched
