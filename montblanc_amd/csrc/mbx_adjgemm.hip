@@ -49,9 +49,8 @@ static __host__ __device__ inline size_t adjg_setup_bytes(int na) {
 }
 
 template <typename T>
-static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, int split = 1) {
+static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw) {
     return adjg_setup_bytes(na) + 2 * size_t(rows) * rows * sizeof(T)   // W planes
-           + size_t(split - 1) * st * na * sizeof(J4<T>) + 16            // partial Jb (WIDE)
            + 2 * size_t(2 * st) * rows * sizeof(T)                      // X planes (then Jb)
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
            + size_t(st) * 4 * sizeof(double)                            // l, m, n - 1
@@ -134,9 +133,11 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     // and the antennas' u, v [2][na]
     T *Lxy = Lst + st * 16;
     T *Luv = Lxy + 2 * NA * st;
-    // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na]
-    J4<T> *Jpart = reinterpret_cast<J4<T> *>(
-        smem_raw + ((reinterpret_cast<unsigned char *>(Luv + 2 * NA) - smem_raw + 15) & ~size_t(15)));
+    // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na],
+    // written over X once every X read is done (they fit: (kSplit - 1) st na
+    // J4 <= 2 planes x 2 st x rows for kSplit = 2, since 2 na <= rows)
+    static_assert(kSplit <= 2, "the partials are kept over X");
+    J4<T> *Jpart = Jb;
 
     tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kThreads);
     stage_antennas<T>(R, Lant, t, c, tid, kThreads);
@@ -795,13 +796,13 @@ static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds, int thre
     S.st = std::min(16, (threads / split / na) / 8 * 8);
     if (S.st < 8) return false;
     static size_t budget = gemm_lds_budget();
-    auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw, split) <= budget; };
+    auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw) <= budget; };
     if (!fits(S.st, S.st) && S.st > 8) S.st = 8;
     if (!fits(S.st, S.st)) return false;
     // the widest flush window that fits, up to 32 tiles
     S.sw = S.st;
     while (S.sw < 32 * S.st && fits(S.st, S.sw + S.st)) S.sw += S.st;
-    lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw, split);
+    lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw);
     return (S.rows / 16) * (S.st / 8) <= kAdjMfmaWaves * kAdjMaxTiles;
 }
 
