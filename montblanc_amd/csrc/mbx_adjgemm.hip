@@ -1,31 +1,27 @@
-// Source stage of the adjoint on the matrix cores, one workgroup per
-// (time t, channel c). No reference counterpart.
+// Source stage of the adjoint, one workgroup per (time t, channel c). No
+// reference counterpart.
 //
 // With Vb the visibility adjoint (from k_adj_vis), the Jones adjoint of
 // antenna p for source s is
 //     Jb_ps = sum_q f_pq,s W_pq J_qs,
 // where W is the 2na x 2na Hermitian completion of Vb: W_pq = Vb_b and
 // W_qp = Vb_b^H for baseline b = (p, q), W_pp = Vb_b + Vb_b^H for an
-// auto-correlation. The workgroup stages W in LDS once (planes Re, Im).
+// auto-correlation.
 //
-// Point pass (f = 1): Jb = W X is a matrix product, X[2q+k][2s+j] =
+// Point sources (f = 1): Jb = W X is a matrix product, X[2q+k][2s+j] =
 // J_qs[k][j]; in real arithmetic Re Jb = [Re W | -Im W] [Re X; Im X] and
 // Im Jb = [Im W | Re W] [Re X; Im X], on v_mfma_*_16x16x4 with 16x16
-// output tiles (antenna rows x (source, column)).
-// Shaped pass (gaussian, sersic): one thread per (source, antenna p)
-// walks q on the vector units, with W and J from LDS, and also forms the
-// shape-factor adjoint d chi^2/d f_pq = Re <J_p, W_pq J_q> (summed over
-// both orders of a pair, this is 2 Re <J_p, Vb J_q> per baseline).
-// Both passes then run the per-(source, antenna) backward of J = E K
-// sqrt(B) (antenna_backward) and reduce each source's gradient in LDS
-// before one set of float64 atomics per source and workgroup.
-//
-// Per source tile: source parameters | X (all waves, one (source,
-// antenna) item each, whose forward state -- beam Jones and its l, m
-// derivatives, phase, sqrt(B0) -- stays in registers) | Jb (MFMA waves,
-// or all waves) | backward from the kept state | flush per window, with a
-// barrier between the phases. Jb is held in registers across the barrier
-// that ends the product and then written over X's LDS.
+// output tiles (antenna rows x (source, column)). k_adj_point pipelines
+// it over source tiles with W in the matrix waves' registers;
+// k_adj_gemm<T, false> (W in LDS, one phase per tile) is the fallback.
+// Shaped sources (gaussian, sersic), k_adj_gemm<T, true, WIDE>: one thread
+// per (source, antenna p) walks q on the vector units, with W and J from
+// LDS, and also forms the shape-factor adjoint d chi^2/d f_pq =
+// Re <J_p, W_pq J_q> (summed over both orders of a pair, this is
+// 2 Re <J_p, Vb J_q> per baseline).
+// Every pass then runs the per-(source, antenna) backward of J = E K
+// sqrt(B) and reduces each source's gradient in LDS before one set of
+// float64 atomics per source and window of source tiles.
 
 #include "mbx_backward.hpp"
 #include "mbx_gemm.hpp"
@@ -58,12 +54,8 @@ static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw) {
            + (2 * size_t(na) * st + 2 * size_t(na)) * sizeof(T);        // shape coordinates, u, v
 }
 
-// Shape factors of a pair from r2 = dx^2 + dy^2: gaussian exp(-r2) (float:
-// the hardware exp2), sersic (1 + r2)^(-3/2) with id = 1 / (1 + r2).
-static __device__ __forceinline__ float gauss_factor(float r2) {
-    return __builtin_amdgcn_exp2f(-1.4426950408889634f * r2);
-}
-static __device__ __forceinline__ double gauss_factor(double r2) { return exp(-r2); }
+// Sersic factor of a pair from r2 = dx^2 + dy^2: (1 + r2)^(-3/2), with
+// id = 1 / (1 + r2) (the gaussian one is gauss_of, mbx_tile.hpp).
 static __device__ __forceinline__ float sersic_factor(float r2, float &id) {
     const float rs = __builtin_amdgcn_rsqf(1.0f + r2);
     id = rs * rs;
@@ -379,7 +371,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         const J4<T> Y = pair_y(q);
                         const T dx = Lx[lo] - xp, dy = Ly[lo] - yp;
                         advance(q);
-                        const T f = gauss_factor(fma(dx, dx, dy * dy));
+                        const T f = gauss_of(fma(dx, dx, dy * dy));
                         accumulate(f, Y);
                         if (q > p) {
                             const T du = Luv[q] - up, dv = Luv[NA + q] - vp;
