@@ -22,7 +22,7 @@ HBM bytes of the fused forward are the compulsory array traffic: every
 input read once, model_vis and chi_sqrd_result written once.
 """
 
-PEAK_TFLOPS = {'f32': 157.3, 'f64': 78.6}    # MI355X vector FP32 (packed) / FP64
+PEAK_TFLOPS = {'f32': 157.3, 'f64': 78.6}    # MI355X dense FP32 / FP64 (matrix cores; packed-vector FP32 and vector FP64 are the same figures)
 PEAK_TFLOPS_UNPACKED_F32 = 78.6              # FP32 without packed-math instructions
 PEAK_HBM_GBS = 8000.0
 
