@@ -127,7 +127,12 @@ class _PerSample(object):
         nchan = a['frequency'].shape[0]
         dev = batch.device
         ct = rime.complex_dtype(a)
-        ngroups = max(1, min(self.STREAMS, len(batch.counts)))
+        # each stream holds its own scratch (model and pre-gain sums, chi^2
+        # terms, and a workspace of about one more visibility array): at
+        # most a quarter of the free device memory goes to them
+        per_group = ntime * nbl * nchan * (4 * ct.itemsize * (3 if gradients else 2) + ft.itemsize)
+        free = torch.cuda.mem_get_info(dev)[0]
+        ngroups = max(1, min(self.STREAMS, len(batch.counts), int(0.25 * free // max(per_group, 1))))
         self.streams = [torch.cuda.Stream(dev) for _ in range(ngroups)]
         self.workspaces = [rime.Workspace(dev) for _ in range(ngroups)]
         outs = []
