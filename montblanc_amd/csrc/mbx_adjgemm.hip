@@ -436,7 +436,11 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                                           an[3], c, Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1], o[0], o[1],
                                           o[2], o[3], o[4], ist);
                     double g[7];
-                    item_backward<T>(ist, acc, g);
+                    item_backward<T>(ist,
+                                     item_chain<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[2], an[3],
+                                                   Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1], ist.okl,
+                                                   ist.okm),
+                                     acc, g);
 #pragma unroll
                     for (int x = 0; x < 7; ++x) atomicAdd(sac + si * 10 + x, g[x]);
                 }
@@ -446,7 +450,11 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         // ---- backward of J = E K sqrt(B) per (source, antenna)
         if (own && !WIDE) {
             double g[7];
-            item_backward<T>(ist, Jb[isi * NA + ia], g);
+            const double *an = Lant + 4 * ia;
+            item_backward<T>(ist,
+                             item_chain<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[2], an[3], Lch[3],
+                                           Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], ist.okl, ist.okm),
+                             Jb[isi * NA + ia], g);
 #pragma unroll
             for (int x = 0; x < 7; ++x) atomicAdd(sac + isi * 10 + x, g[x]);
         }
@@ -659,7 +667,11 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
                                           an[3], c, Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si],
                                           T(prm(it, 3)[si]), T(prm(it, 4)[si]), T(prm(it, 5)[si]),
                                           T(prm(it, 6)[si]), T(prm(it, 7)[si]), ist);
-                    item_backward<T>(ist, Jb, gr);
+                    item_backward<T>(ist,
+                                     item_chain<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[2], an[3],
+                                                   Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si], ist.okl,
+                                                   ist.okm),
+                                     Jb, gr);
                     double *sa = sacc + ((it / S.wt) & 1) * wsrc * 7 + ((it % S.wt) * st + si) * 7;
 #pragma unroll
                     for (int x = 0; x < 7; ++x) atomicAdd(sa + x, gr[x]);

@@ -57,15 +57,38 @@ __device__ __forceinline__ T jdot_re(const J4<T> &A, const J4<T> &B) {
 // ---------------------------------------------------------------------
 // Per-item backward of J = E K sqrt(B) onto (l, m, I, Q, U, V, alpha).
 
-// What the backward of one (source, antenna, channel) item needs from its
+// What the backward of one (source, antenna, channel) item keeps from its
 // forward: the beam Jones and its derivatives along l and m, the phase
-// term, sqrt(B0) of the unscaled brightness and the chain-rule factors.
+// term, the brightness terms and whether the beam position was inside the
+// grid. What is cheap to recompute from the tile's LDS records (the
+// chain-rule factors, sqrt(B0)) is left out, so that the state held across
+// a source tile's pair loop stays small.
 template <typename T> struct ItemState {
-    J4<T> E, dEl, dEm, B0s;
+    J4<T> E, dEl, dEm;
     cx<T> K;
     T rP, s0, it0, I, Q, U, V;
+    bool okl, okm;
+};
+
+// Chain-rule factors of one item from (l, m) to its phase and its beam
+// grid position, and ln(f / f_ref) (inputs as item_forward).
+struct ItemChain {
     double lnr, dphi_dl, dphi_dm, dvl_dl, dvl_dm, dvm_dl, dvm_dm;
 };
+
+template <typename T>
+__device__ __forceinline__ ItemChain item_chain(const RimeArgs<T> &A, const double *xa, double ps,
+                                                double pc, double al, double am, double f, double lnr,
+                                                double l0, double m0, bool okl, bool okm) {
+    const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
+    ItemChain ch;
+    ch.lnr = lnr;
+    ch.dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * l0 / n);
+    ch.dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * m0 / n);
+    ch.dvl_dl = okl ? A.lscale * al * pc : 0.0; ch.dvl_dm = okl ? -A.lscale * al * ps : 0.0;
+    ch.dvm_dl = okm ? A.mscale * am * ps : 0.0; ch.dvm_dm = okm ? A.mscale * am * pc : 0.0;
+    return ch;
+}
 
 // Forward of one item with the derivative information its backward needs.
 // Inputs: the source's l, m; its Stokes I, Q, U, V and spectral power
@@ -159,26 +182,26 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
     J4<T> X;
 #pragma unroll
     for (int x = 0; x < 4; ++x) X.v[x] = cmul(K, sB.v[x]);
-    st.E = E; st.dEl = dEl; st.dEm = dEm; st.B0s = B0s; st.K = K;
+    st.E = E; st.dEl = dEl; st.dEm = dEm; st.K = K;
     st.rP = rP; st.s0 = s0; st.it0 = it0; st.I = I; st.Q = Q; st.U = U; st.V = V;
-    st.lnr = lnr;
-    st.dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * l0 / n);
-    st.dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * m0 / n);
-    st.dvl_dl = okl ? A.lscale * al * pc : 0.0; st.dvl_dm = okl ? -A.lscale * al * ps : 0.0;
-    st.dvm_dl = okm ? A.mscale * am * ps : 0.0; st.dvm_dm = okm ? A.mscale * am * pc : 0.0;
+    st.okl = okl; st.okm = okm;
     return jmul(E, X);
 }
 
 // Backward of one item: d chi^2 / d(l, m, I, Q, U, V, alpha) into g from
-// the Jones adjoint Jb and the item's forward state.
+// the Jones adjoint Jb, the item's forward state and its chain-rule
+// factors (item_chain).
 template <typename T>
-__device__ __forceinline__ void item_backward(const ItemState<T> &st, const J4<T> &Jb,
-                                              double *g /* [7] */) {
-    const J4<T> &E = st.E, &dEl = st.dEl, &dEm = st.dEm, &B0s = st.B0s;
+__device__ __forceinline__ void item_backward(const ItemState<T> &st, const ItemChain &ch,
+                                              const J4<T> &Jb, double *g /* [7] */) {
+    const J4<T> &E = st.E, &dEl = st.dEl, &dEm = st.dEm;
     const cx<T> K = st.K;
     const T rP = st.rP, s0 = st.s0, it0 = st.it0, I = st.I, Q = st.Q, U = st.U, V = st.V;
     const T t0 = it0 != T(0) ? T(1) : T(0);   // nonzero marker: it0 = 1/t0 or 0
-    const double lnr = st.lnr;
+    const double lnr = ch.lnr;
+    J4<T> B0s;                                // sqrt(B0), as item_forward
+    B0s.v[0] = {I + Q + s0, T(0)}; B0s.v[1] = {U, V}; B0s.v[2] = {U, -V}; B0s.v[3] = {I - Q + s0, T(0)};
+    if (t0 != T(0)) B0s = jscale(B0s, it0);
     const J4<T> sB = jscale(B0s, rP);
     J4<T> X;
 #pragma unroll
@@ -218,8 +241,8 @@ __device__ __forceinline__ void item_backward(const ItemState<T> &st, const J4<T
         g[2] = g[3] = g[4] = g[5] = 0.0;
     }
     // l, m: phase and beam-weight paths
-    g[0] = double(g_phi) * st.dphi_dl + double(g_ld) * st.dvl_dl + double(g_md) * st.dvm_dl;
-    g[1] = double(g_phi) * st.dphi_dm + double(g_ld) * st.dvl_dm + double(g_md) * st.dvm_dm;
+    g[0] = double(g_phi) * ch.dphi_dl + double(g_ld) * ch.dvl_dl + double(g_md) * ch.dvm_dl;
+    g[1] = double(g_phi) * ch.dphi_dm + double(g_ld) * ch.dvl_dm + double(g_md) * ch.dvm_dm;
 }
 
 // Forward and backward of one item in one call (inputs as item_forward).
@@ -232,7 +255,7 @@ __device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const 
                                                     double *g /* [7] */) {
     ItemState<T> st;
     (void)item_forward<T>(A, xa, ps, pc, pel, pem, al, am, c, f, lnr, l0, m0, I, Q, U, V, P, st);
-    item_backward<T>(st, Jb, g);
+    item_backward<T>(st, item_chain<T>(A, xa, ps, pc, al, am, f, lnr, l0, m0, st.okl, st.okm), Jb, g);
 }
 
 // The same item reading its inputs from global memory and the tile's
