@@ -174,9 +174,12 @@ template <typename T> struct Tile {
         ch = pa + 2 * NA;
         src = ch + CH * ct;
         shape = reinterpret_cast<T *>(src + 3 * st);
-        uintptr_t off = reinterpret_cast<uintptr_t>(shape + 4 * st);
-        off = (off + 15) & ~uintptr_t(15);
-        sqB = reinterpret_cast<J4<T> *>(off);
+        // 16-byte aligned offset from the LDS base: the pointers stay
+        // derived from the shared array, so their accesses compile to LDS
+        // instructions (a round trip through an integer address would
+        // make them generic flat accesses)
+        const size_t off = (size_t(reinterpret_cast<unsigned char *>(shape + 4 * st) - base) + 15) & ~size_t(15);
+        sqB = reinterpret_cast<J4<T> *>(base + off);
         J = sqB + st * ct;
         part = reinterpret_cast<double *>(J + st * NA * ct);
     }
@@ -512,9 +515,9 @@ template <typename T> struct PcTile {
         pa = uvw + 3 * NA;
         ch = pa + 2 * NA;
         shape = reinterpret_cast<T *>(ch + CH * ct);
-        uintptr_t off = reinterpret_cast<uintptr_t>(shape + 2 * 4 * st);
-        off = (off + 15) & ~uintptr_t(15);
-        J = reinterpret_cast<J4<T> *>(off);
+        // aligned offset from the LDS base (see Tile)
+        const size_t off = (size_t(reinterpret_cast<unsigned char *>(shape + 2 * 4 * st) - base) + 15) & ~size_t(15);
+        J = reinterpret_cast<J4<T> *>(base + off);
         jstride = st * NA * ct;
         part = reinterpret_cast<double *>(J + 2 * jstride);
     }
