@@ -342,6 +342,274 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     }
 }
 
+// ---------------------------------------------------------------------
+// Gaussian and sersic sources in float on the matrix cores, 4x4 blocks.
+//
+// The shape factor f_pq(s) differs per element of the coherency matrix,
+// so a 16x16 product over the Jones reals has to be scaled element by
+// element after every source. v_mfma_f32_4x4x1f32 instead runs sixteen
+// independent 4x4 rank-1 updates: lane 4b + i supplies row i of block b's
+// A operand, lane 4b + j column j of its B operand, and output register r
+// of lane 4b + j is element (r, j) of block b. Block b of a consumer wave
+// owns, for each of SETS sets g, the pairs (p_g, q) of one antenna p_g
+// with an antenna quad q = 4Q + j. With the four reals of a Jones row and
+// its rotation,
+//     X row ia = (Re J_ia0, Re J_ia1, Im J_ia0, Im J_ia1),
+//     R row ia = (Im J_ia0, Im J_ia1, -Re J_ia0, -Re J_ia1),
+// lane i supplies (i < 2 ? X row i : R row i - 2)[m] of antenna p_g and
+// lane j supplies f_pq (X row b')[m] of antenna q; summed over m = 0..3,
+//     D_{g,b'}[i][j] = f_pq (Re V[i][b'] for i < 2, Im V[i - 2][b'] else),
+// V = J_p J_q^H. The factor rides on the B operand (along a block's
+// columns only q varies) and the accumulators sum over the sources inside
+// the matrix core: the vector units evaluate f and scale eight values per
+// pair, with no pass over the products afterwards.
+//
+// Strips: the pairs p <= q are covered by (p, quad Q) strips, p in
+// [0, min(4Q + 4, na)); consecutive runs of SETS strips of one quad form
+// a block's group. A host plan picks SETS and the number of consumer waves
+// (4, 8 or 12; producers fill the rest of 1024 threads), at most four
+// sets (eight accumulators each).
+
+constexpr int kMxThreads = 1024;
+constexpr int kMxMaxSets = 4;   // 1024 threads: 128 VGPRs; five sets spill
+
+// antennas p of the strips of quad m
+__host__ __device__ inline int mx_quad_span(int na, int m) { return min(4 * m + 4, na); }
+
+static inline int mx_groups(int na, int sets) {
+    int n = 0;
+    for (int m = 0; 4 * m < na; ++m) n += (mx_quad_span(na, m) + sets - 1) / sets;
+    return n;
+}
+
+// Per buffer: X and R rows of st sources x na antennas plus kMxMaxSets
+// zeroed pad antennas (a block's last set and a partial last quad read
+// up to kMxMaxSets - 1 antennas past the end), the same for the shape
+// coordinates.
+__host__ __device__ inline size_t mx_buf_ants(int na, int st) { return size_t(st) * na + kMxMaxSets; }
+static inline size_t mx_lds_bytes(int na, int st) {
+    return gemm_x_offset(na) + 2 * mx_buf_ants(na, st) * (4 * sizeof(XRow<float>) + sizeof(float2));
+}
+
+struct MxPlan {
+    int sets, waves, st;
+    size_t lds;
+};
+
+// Fewest sets per SIMD (consumer waves per SIMD x sets) over the
+// layouts whose groups fit 16 blocks per consumer wave; the source tile
+// is the largest multiple of four up to 32 that fits the LDS budget.
+static bool mx_plan(int na, size_t budget, MxPlan &p) {
+    int best = 1 << 30;
+    p.sets = p.waves = 0;
+    for (int w : {8, 4, 12}) {
+        for (int g = 1; g <= kMxMaxSets; ++g) {
+            if (mx_groups(na, g) > 16 * w) continue;
+            const int cost = ((w + 3) / 4) * g;
+            if (cost < best) { best = cost; p.sets = g; p.waves = w; }
+            break;
+        }
+    }
+    if (!p.sets) return false;
+    p.st = 32;
+    while (p.st > 4 && mx_lds_bytes(na, p.st) > budget) p.st -= 4;
+    p.lds = mx_lds_bytes(na, p.st);
+    return p.lds <= budget && p.st <= kMxThreads - 64 * p.waves;
+}
+
+// Sources [s0, s1) of one kind into a block's SETS groups of pairs (see
+// above). Xp / XYp address antenna p0 of source s0 (set g reads antenna
+// p0 + g at a fixed offset), Xq / XYq antenna q; one source is na antenna
+// records further on.
+template <int SETS, bool GAUSS>
+__device__ __forceinline__ void mx_sources(const XRow<float> *Xp, const XRow<float> *Xq, const float2 *XYp,
+                                           const float2 *XYq, int na, int s0, int s1, f32x4 (&D)[SETS][2]) {
+    for (int s = s0; s < s1; ++s) {
+        const XRow<float> b0 = Xq[0], b1 = Xq[1];
+        const float2 cq = XYq[0];
+#pragma unroll
+        for (int g = 0; g < SETS; ++g) {
+            const XRow<float> a = Xp[4 * g];
+            const float2 cp = XYp[g];
+            const float f = shape_f<float>(GAUSS, cq.x - cp.x, cq.y - cp.y);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                D[g][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a.v[m], f * b0.v[m], D[g][0], 0, 0, 0);
+                D[g][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a.v[m], f * b1.v[m], D[g][1], 0, 0, 0);
+            }
+        }
+        Xp += 4 * na; Xq += 4 * na; XYp += na; XYq += na;
+    }
+}
+
+template <int SETS>
+__global__ void __launch_bounds__(kMxThreads)
+k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw) {
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    const mbx_dims &d = A.d;
+    const int NA = d.na.local_size;
+    const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
+    const int tid = threadIdx.x;
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
+
+    const int st = G.st;
+    double *Luvw = reinterpret_cast<double *>(smem_raw);
+    double *Lpa = Luvw + 3 * NA;
+    double *Lant = Lpa + 2 * NA;
+    double *Lch = Lant + 4 * NA;
+    // two source-tile buffers: X and R rows per (source, antenna) as
+    // [X row 0, X row 1, R row 0, R row 1], then shape coordinates (x, y)
+    // per (source, antenna)
+    XRow<float> *Xbase = reinterpret_cast<XRow<float> *>(smem_raw + gemm_x_offset(NA));
+    const int bufa = int(mx_buf_ants(NA, st));
+    const int xstride = 4 * bufa;            // rows per buffer
+    float2 *XYbase = reinterpret_cast<float2 *>(Xbase + 2 * xstride);
+
+    tile_setup<float>(A, Luvw, Lpa, Lch, t, c, tid, kMxThreads);
+    stage_antennas<float>(A, Lant, t, c, tid, kMxThreads);
+    // pad antennas of both buffers: finite values for the reads past the end
+    for (int k = tid; k < 2 * kMxMaxSets; k += kMxThreads) {
+        const int b = k / kMxMaxSets, a = st * NA + k % kMxMaxSets;
+        for (int r = 0; r < 4; ++r) Xbase[b * xstride + 4 * a + r] = XRow<float>{{0.f, 0.f, 0.f, 0.f}};
+        XYbase[b * bufa + a] = float2{0.f, 0.f};
+    }
+    __syncthreads();
+
+    const int np = dext(d.npsrc), ns = dext(d.nsrc), ng = dext(d.ngsrc);
+    const int ntiles = (ns - np + st - 1) / st;
+    auto tile_range = [&](int it, int &s0, int &nst) {
+        s0 = np + it * st;
+        nst = min(st, ns - s0);
+    };
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (wave < ncw) {
+        const int lane = tid & 63, blk = lane >> 2, i = lane & 3;
+        // this block's group: quad Q, antennas p0 .. p0 + SETS - 1 (a
+        // block past the last group computes quad 0 and stores nothing)
+        int gi = wave * 16 + blk, Q = -1, p0 = 0;
+        for (int m = 0; 4 * m < NA; ++m) {
+            const int n = (mx_quad_span(NA, m) + SETS - 1) / SETS;
+            if (gi < n) { Q = m; p0 = gi * SETS; break; }
+            gi -= n;
+        }
+        const bool active = Q >= 0;
+        if (!active) Q = 0;
+        const int q = 4 * Q + i;                 // < na + 3: inside the pad
+        const bool qok = active && q < NA;
+        const int span = active ? mx_quad_span(NA, Q) : 0;
+        // groups end inside the quad's span, so p0 + SETS - 1 < na + SETS - 1
+        f32x4 D[SETS][2];
+#pragma unroll
+        for (int g = 0; g < SETS; ++g) D[g][0] = D[g][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        __syncthreads();                      // tile 0 produced
+        for (int it = 0; it < ntiles; ++it) {
+            const int buf = it & 1;
+            int s0t, nst;
+            tile_range(it, s0t, nst);
+            const XRow<float> *Xb = Xbase + buf * xstride;
+            const float2 *XY = XYbase + buf * bufa;
+            // gaussian sources precede the sersic ones: one loop per kind
+            const int ngt = min(max(np + ng - s0t, 0), nst);
+            mx_sources<SETS, true>(Xb + 4 * p0 + i, Xb + 4 * q, XY + p0, XY + q, NA, 0, ngt, D);
+            mx_sources<SETS, false>(Xb + 4 * (ngt * NA + p0) + i, Xb + 4 * (ngt * NA + q),
+                                    XY + ngt * NA + p0, XY + ngt * NA + q, NA, ngt, nst, D);
+            __syncthreads();
+        }
+        // epilogue: the (p <= q) pairs to their baselines, as the point pass
+        const uint64_t NC = d.nchan.local_size, NBL = d.nbl.local_size;
+#pragma unroll
+        for (int g = 0; g < SETS; ++g) {
+            const int p = p0 + g;
+            if (!qok || p >= span || p > q) continue;
+            const int e = G.bmap[p * NA + q];
+            if (e < 0) continue;
+            const bool rev = (e & MBX_BL_REVERSED) != 0;
+            const int b = e & ~(MBX_BL_REVERSED | MBX_BL_AUTO);
+            if (uint64_t(b) >= NBL) continue;   // a map for another layout
+            J4<float> V;
+#pragma unroll
+            for (int ip = 0; ip < 2; ++ip)
+#pragma unroll
+                for (int kq = 0; kq < 2; ++kq) {
+                    const float re = D[g][kq][ip], im = D[g][kq][2 + ip];
+                    V.v[rev ? 2 * kq + ip : 2 * ip + kq] = cx<float>{re, rev ? -im : im};
+                }
+            const uint64_t vi = ((uint64_t(t) * NBL + b) * NC + c) * 4;
+            if (G.vin) {
+                const J4<float> o = jload(G.vin + vi);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { V.v[k].re += o.v[k].re; V.v[k].im += o.v[k].im; }
+            }
+            jstore(G.dst + vi, V);
+        }
+    } else {
+        const int ptid = tid - 64 * ncw, pth = kMxThreads - 64 * ncw;
+        const uint64_t NGS = d.ngsrc.local_size, NSS = d.nssrc.local_size;
+        auto produce = [&](int it, int buf) {
+            int s0, nst;
+            tile_range(it, s0, nst);
+            XRow<float> *Xb = Xbase + buf * xstride;
+            float2 *XY = XYbase + buf * bufa;
+            // a group of consecutive threads shares a source and walks the antennas
+            const int gs = pth / nst;
+            const int si = ptid / gs, a0 = ptid % gs;
+            if (si >= nst) return;
+            const int s = s0 + si;
+            double l0, m0, nm1;
+            const J4<float> B = source_sqrtb<float>(A, Lch, t, s, l0, m0, nm1);
+            // shape coordinates: linear in the antenna's (u, v)
+            double cu, cv, du, dv;        // x = cu u + cv v, y = du u + dv v
+            const bool gauss = s < np + ng;
+            if (gauss) {
+                const int gi = s - np;
+                const double el = A.gauss[gi], em = A.gauss[gi + NGS], R = A.gauss[gi + 2 * NGS];
+                const double k = A.gauss_scale * Lch[3] * 1.2011224087864498;   // sqrt(log2 e): exp2
+                cu = k * R * em; cv = -k * R * el; du = k * el; dv = k * em;
+            } else {
+                const int si2 = s - np - ng;
+                const double e1 = A.sersic[si2], e2 = A.sersic[si2 + NSS];
+                const double Rp = A.sersic[si2 + 2 * NSS] / (1.0 - e1 * e1 - e2 * e2);
+                const double k = A.two_pi_over_c * Lch[3] * Rp;
+                cu = k * (1.0 + e1); cv = k * e2; du = k * e2; dv = k * (1.0 - e1);
+            }
+            for (int a = a0; a < NA; a += gs) {
+                const J4<float> J =
+                    staged_jones<float, kGemmTaps<float>()>(A, Luvw, Lpa, Lant, Lch, c, a, l0, m0, nm1, B);
+                XRow<float> *r = Xb + 4 * (si * NA + a);
+                r[0] = XRow<float>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
+                r[1] = XRow<float>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
+                r[2] = XRow<float>{{J.v[0].im, J.v[1].im, -J.v[0].re, -J.v[1].re}};
+                r[3] = XRow<float>{{J.v[2].im, J.v[3].im, -J.v[2].re, -J.v[3].re}};
+                const double u = Luvw[3 * a], v = Luvw[3 * a + 1];
+                XY[si * NA + a] = float2{float(cu * u + cv * v), float(du * u + dv * v)};
+            }
+        };
+        if (ntiles > 0) produce(0, 0);
+        __syncthreads();
+        for (int it = 0; it < ntiles; ++it) {
+            if (it + 1 < ntiles) produce(it + 1, (it + 1) & 1);
+            __syncthreads();
+        }
+    }
+}
+
+static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_t nwg, hipStream_t stream) {
+    const int na = A.d.na.local_size;
+    static size_t budget = gemm_lds_budget();
+    MxPlan p;
+    if (!mx_plan(na, budget, p)) return set_error(MBX_E_UNSUPPORTED, "shaped matrix pass: %d antennas", na);
+    G.st = p.st;
+    const dim3 grid{unsigned(nwg)}, block{unsigned(kMxThreads)};
+    switch (p.sets) {
+    case 1: hipLaunchKernelGGL((k_shaped_mx<1>), grid, block, p.lds, stream, A, G, p.waves); break;
+    case 2: hipLaunchKernelGGL((k_shaped_mx<2>), grid, block, p.lds, stream, A, G, p.waves); break;
+    case 3: hipLaunchKernelGGL((k_shaped_mx<3>), grid, block, p.lds, stream, A, G, p.waves); break;
+    default: hipLaunchKernelGGL((k_shaped_mx<4>), grid, block, p.lds, stream, A, G, p.waves); break;
+    }
+    return hip_status(hipGetLastError(), "shaped matrix pass launch");
+}
+
 template <typename T>
 bool point_gemm_supported(const mbx_dims &d) {
     const int na = d.na.local_size;
@@ -350,13 +618,17 @@ bool point_gemm_supported(const mbx_dims &d) {
 
 template <typename T>
 bool shaped_gemm_pays(const mbx_dims &d) {
-    // Measured on MI355X: the per-element shape factors keep the shaped
-    // pass VALU-bound; it beats the vector-unit kernel only in float64
-    // with many antennas (measured at 64 antennas with 600 gaussian +
-    // sersic sources). In float32 the vector-unit kernel stays ahead (64
-    // antennas: whole forward 28.6 ms against 31.5 ms), as it does at 27
-    // antennas.
-    return sizeof(T) == 8 && d.na.local_size >= 48;
+    // Measured on MI355X at 64 antennas with 600 gaussian + sersic
+    // sources: float, the 4x4-block pass (k_shaped_mx) 11.6 ms against
+    // 14.9 ms for the vector-unit kernel; float64, the 16x16 pass with
+    // per-element shape factors ahead of the vector-unit kernel. They are
+    // used from 40 (float) and 48 (float64) antennas up: fewer antennas
+    // give fewer pairs per source to fill the matrix cores.
+    if (sizeof(T) == 4) {
+        MxPlan p;
+        return d.na.local_size >= 40 && mx_plan(d.na.local_size, gemm_lds_budget(), p);
+    }
+    return d.na.local_size >= 48;
 }
 
 template <typename T>
@@ -387,6 +659,7 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
         G.vin = dst;   // the shaped pass continues from the point sums
     }
     if (shaped && ns > np) {
+        if constexpr (sizeof(T) == 4) return launch_shaped_mx(Ag, G, nwg, stream);
         constexpr int cw = gemm_consumers<true>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
         if (G.tpw > gemm_max_tp<true>()) return set_error(MBX_E_UNSUPPORTED, "shaped gemm: %d antennas", na);

@@ -62,3 +62,14 @@ def test_too_many_antennas_falls_back(dtype):
     counts = (3, 1, 0)
     a = synth.make_inputs(65, 1, 2, *counts, beam=(9, 8, 6), seed=8)
     check(a, counts, dtype)
+
+
+@pytest.mark.parametrize('na', [41, 64])
+def test_shaped_matrix_pass_many_sources(na):
+    """Gaussian and sersic sources on the matrix cores in float (40
+    antennas and up): several source tiles, the gaussian / sersic boundary
+    inside a tile, a partial last antenna quad (41 antennas), point
+    sources before them and a flagged fraction."""
+    counts = (3, 37, 29)
+    a = synth.make_inputs(na, 2, 2, *counts, beam=(9, 8, 6), seed=na, flag_fraction=0.1)
+    check(a, counts, np.float32)
