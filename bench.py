@@ -375,17 +375,20 @@ def main():
         'roofline': {'bound': 'mfma', 'achieved': r['tflops'], 'peak': peak,
                      'unit': 'TFLOP/s', 'frac': r['tflops'] / peak,
                      'traffic': measured_traffic(args.workload, primary),
-                     'kernel': 'mbx_forward_%s (k_beam_planes, k_point_gemm, k_forward_pc, '
-                               'k_chi2_final)' % primary,
+                     'kernel': 'mbx_forward_%s (k_beam_planes, k_point_gemm, k_shaped_mx, '
+                               'k_forward_pc epilogue, k_chi2_final)' % primary,
                      'flops_per_launch': r['flops'],
-                     'peak_note': 'compute-bound: point sources on the matrix cores, gaussian and '
-                                  'sersic sources on the vector units; priced against the dense '
-                                  'peak of the dtype (FP32 157.3 TFLOP/s on the matrix cores, the '
-                                  'same figure as the packed-FP32 vector peak; this build compiles '
-                                  'packed FP32 out, leaving a 78.6 TFLOP/s vector rate)',
-                     'limiters': 'k_forward_pc: vector issue ~100% of cycles; k_point_gemm: '
-                                 'texture-data path 63%, matrix cores 53%, vector units 45% '
-                                 '(rocprofv3 PMC, forward_64ant f32)'},
+                     'peak_note': 'compute-bound: point sources on the matrix cores (16x16x4 '
+                                  'products), gaussian and sersic sources on the matrix cores '
+                                  '(4x4x1 blocks, shape factor on the B operand); priced against '
+                                  'the dense peak of the dtype (FP32 157.3 TFLOP/s on the matrix '
+                                  'cores, the same figure as the packed-FP32 vector peak; this '
+                                  'build compiles packed FP32 out, leaving a 78.6 TFLOP/s vector '
+                                  'rate)',
+                     'limiters': 'k_point_gemm: texture-data path 63%, matrix cores 53%, vector '
+                                 'units 45%; k_shaped_mx: matrix cores 32%, vector issue 68% of '
+                                 'busy cycles, LDS bank conflicts 13% (rocprofv3 PMC, '
+                                 'forward_64ant f32)'},
         'per_dtype': {k: {'terms_per_s': v['terms_per_s'], 'ms_per_step': v['ms_per_step'],
                           'ms_event_mean': v['ms_event_mean'], 'tflops': v['tflops'],
                           'frac_of_peak': v['tflops'] / roofline.PEAK_TFLOPS[k], 'X2': v['X2']}
