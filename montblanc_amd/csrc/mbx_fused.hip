@@ -623,9 +623,21 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
         // point sources (and where it pays the gaussian and sersic ones)
         // on the matrix cores into the running-sum buffer; the kernel
         // below adds the remaining sources and applies gains, flags, chi^2
+        // -- unless the float shaped pass, the last one, applied them
         cx<T> *vbuf = A.ekb ? A.ekb : A.model;
-        r = launch_point_gemm<T>(A, pp->baseline_map, vbuf, A.vin, shaped, stream);
+        if (gemm_workgroups(*d) > max_partials(*d))
+            return set_error(MBX_E_UNSUPPORTED, "forward: %llu workgroups exceed the partial buffer",
+                             (unsigned long long)gemm_workgroups(*d));
+        uint64_t epi_wgs = 0;
+        r = launch_point_gemm<T>(A, pp->baseline_map, vbuf, A.vin, shaped, stream, &epi_wgs);
         if (r) return r;
+        if (epi_wgs) {
+            if (final_batch) {
+                launch_chi2_final(partial, unsigned(epi_wgs), scale, out->X2, stream);
+                r = hip_status(hipGetLastError(), "forward chi2 launch");
+            }
+            return r;
+        }
         A.vin = vbuf;
         A.s_begin = shaped ? dext(d->nsrc) : dext(d->npsrc);
     }

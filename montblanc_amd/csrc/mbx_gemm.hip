@@ -385,10 +385,11 @@ static inline int mx_groups(int na, int sets) {
 // Per buffer: X and R rows of st sources x na antennas plus kMxMaxSets
 // zeroed pad antennas (a block's last set and a partial last quad read
 // up to kMxMaxSets - 1 antennas past the end), the same for the shape
-// coordinates.
+// coordinates; then one chi^2 partial per wave.
 __host__ __device__ inline size_t mx_buf_ants(int na, int st) { return size_t(st) * na + kMxMaxSets; }
 static inline size_t mx_lds_bytes(int na, int st) {
-    return gemm_x_offset(na) + 2 * mx_buf_ants(na, st) * (4 * sizeof(XRow<float>) + sizeof(float2));
+    return gemm_x_offset(na) + 2 * mx_buf_ants(na, st) * (4 * sizeof(XRow<float>) + sizeof(float2)) +
+           (kMxThreads / 64) * sizeof(double);   // per-wave chi^2 partials
 }
 
 struct MxPlan {
@@ -442,16 +443,22 @@ __device__ __forceinline__ void mx_sources(const XRow<float> *Xp, const XRow<flo
     }
 }
 
+// epi = 0: the sums go to G.dst (plus G.vin) as in the point pass; epi =
+// 1: the epilogue is applied here (vis_out) and a chi^2 partial per
+// workgroup written to A.partial.
 template <int SETS>
 __global__ void __launch_bounds__(kMxThreads)
-k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw) {
+k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
     extern __shared__ __align__(32) unsigned char smem_raw[];
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const int tid = threadIdx.x;
     int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) {
+        if (epi && tid == 0) A.partial[blockIdx.x] = 0.0;   // padding workgroup
+        return;
+    }
 
     const int st = G.st;
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -465,6 +472,8 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw) {
     const int bufa = int(mx_buf_ants(NA, st));
     const int xstride = 4 * bufa;            // rows per buffer
     float2 *XYbase = reinterpret_cast<float2 *>(Xbase + 2 * xstride);
+    double *Lpart = reinterpret_cast<double *>(XYbase + 2 * bufa);
+    double term_sum = 0.0;
 
     tile_setup<float>(A, Luvw, Lpa, Lch, t, c, tid, kMxThreads);
     stage_antennas<float>(A, Lant, t, c, tid, kMxThreads);
@@ -516,8 +525,10 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw) {
                                     XY + ngt * NA + p0, XY + ngt * NA + q, NA, ngt, nst, D);
             __syncthreads();
         }
-        // epilogue: the (p <= q) pairs to their baselines, as the point pass
+        // the (p <= q) pairs to their baselines (reversed baselines get the
+        // conjugate transpose): running sums, or with epi the epilogue
         const uint64_t NC = d.nchan.local_size, NBL = d.nbl.local_size;
+        const bool final_batch = d.nsrc.upper_extent >= d.nsrc.global_size;
 #pragma unroll
         for (int g = 0; g < SETS; ++g) {
             const int p = p0 + g;
@@ -541,7 +552,10 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) { V.v[k].re += o.v[k].re; V.v[k].im += o.v[k].im; }
             }
-            jstore(G.dst + vi, V);
+            if (epi)   // the baseline's own antenna order: (q, p) when reversed
+                term_sum += vis_out(A, t, b, c, rev ? q : p, rev ? p : q, V, final_batch);
+            else
+                jstore(G.dst + vi, V);
         }
     } else {
         const int ptid = tid - 64 * ncw, pth = kMxThreads - 64 * ncw;
@@ -592,9 +606,11 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw) {
             __syncthreads();
         }
     }
+    if (epi) wg_partial(term_sum, Lpart, A.partial + blockIdx.x, tid, kMxThreads);
 }
 
-static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_t nwg, hipStream_t stream) {
+static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_t nwg, int epi,
+                            hipStream_t stream) {
     const int na = A.d.na.local_size;
     static size_t budget = gemm_lds_budget();
     MxPlan p;
@@ -602,10 +618,10 @@ static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_
     G.st = p.st;
     const dim3 grid{unsigned(nwg)}, block{unsigned(kMxThreads)};
     switch (p.sets) {
-    case 1: hipLaunchKernelGGL((k_shaped_mx<1>), grid, block, p.lds, stream, A, G, p.waves); break;
-    case 2: hipLaunchKernelGGL((k_shaped_mx<2>), grid, block, p.lds, stream, A, G, p.waves); break;
-    case 3: hipLaunchKernelGGL((k_shaped_mx<3>), grid, block, p.lds, stream, A, G, p.waves); break;
-    default: hipLaunchKernelGGL((k_shaped_mx<4>), grid, block, p.lds, stream, A, G, p.waves); break;
+    case 1: hipLaunchKernelGGL((k_shaped_mx<1>), grid, block, p.lds, stream, A, G, p.waves, epi); break;
+    case 2: hipLaunchKernelGGL((k_shaped_mx<2>), grid, block, p.lds, stream, A, G, p.waves, epi); break;
+    case 3: hipLaunchKernelGGL((k_shaped_mx<3>), grid, block, p.lds, stream, A, G, p.waves, epi); break;
+    default: hipLaunchKernelGGL((k_shaped_mx<4>), grid, block, p.lds, stream, A, G, p.waves, epi); break;
     }
     return hip_status(hipGetLastError(), "shaped matrix pass launch");
 }
@@ -631,10 +647,15 @@ bool shaped_gemm_pays(const mbx_dims &d) {
     return d.na.local_size >= 48;
 }
 
+uint64_t gemm_workgroups(const mbx_dims &d) {
+    return 8 * uint64_t(tc_per_xcd(dext(d.ntime), dext(d.nchan)));
+}
+
 template <typename T>
 int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, const cx<T> *vin,
-                      bool shaped, hipStream_t stream) {
+                      bool shaped, hipStream_t stream, uint64_t *epilogue_wgs) {
     const mbx_dims &d = A.d;
+    if (epilogue_wgs) *epilogue_wgs = 0;
     const int na = d.na.local_size;
     GemmArgs<T> G;
     G.bmap = bmap; G.dst = dst; G.vin = vin;
@@ -643,7 +664,7 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
     G.ntp = nR * (nR + 1) / 2;
     static size_t budget = gemm_lds_budget();
     G.st = gemm_st<T>(na, budget);
-    const uint64_t nwg = 8 * uint64_t(tc_per_xcd(dext(d.ntime), dext(d.nchan)));
+    const uint64_t nwg = gemm_workgroups(d);
     RimeArgs<T> Ag = A;
     Ag.ct = 1;
     const size_t lds = gemm_lds_bytes<T>(na, G.st);
@@ -659,7 +680,11 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
         G.vin = dst;   // the shaped pass continues from the point sums
     }
     if (shaped && ns > np) {
-        if constexpr (sizeof(T) == 4) return launch_shaped_mx(Ag, G, nwg, stream);
+        if constexpr (sizeof(T) == 4) {
+            const int r = launch_shaped_mx(Ag, G, nwg, epilogue_wgs ? 1 : 0, stream);
+            if (!r && epilogue_wgs) *epilogue_wgs = nwg;
+            return r;
+        }
         constexpr int cw = gemm_consumers<true>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
         if (G.tpw > gemm_max_tp<true>()) return set_error(MBX_E_UNSUPPORTED, "shaped gemm: %d antennas", na);
@@ -675,8 +700,8 @@ template bool point_gemm_supported<double>(const mbx_dims &);
 template bool shaped_gemm_pays<float>(const mbx_dims &);
 template bool shaped_gemm_pays<double>(const mbx_dims &);
 template int launch_point_gemm<float>(const RimeArgs<float> &, const int32_t *, cx<float> *,
-                                      const cx<float> *, bool, hipStream_t);
+                                      const cx<float> *, bool, hipStream_t, uint64_t *);
 template int launch_point_gemm<double>(const RimeArgs<double> &, const int32_t *, cx<double> *,
-                                       const cx<double> *, bool, hipStream_t);
+                                       const cx<double> *, bool, hipStream_t, uint64_t *);
 
 }  // namespace mbx

@@ -110,8 +110,18 @@ template <typename T> bool shaped_gemm_pays(const mbx_dims &d);
 // dst[t, b, c] = (vin ? vin[t, b, c] : 0) + sum over the point sources
 // [0, npsrc extent) of J_p J_q^H -- and with `shaped` over the gaussian
 // and sersic sources too, f_pq J_p J_q^H -- for every baseline of the map.
+//
+// With `epilogue_wgs` set, a float shaped pass (k_shaped_mx) also applies
+// the epilogue (gains, flags, model or residual, chi^2 terms; A's output
+// arrays) and writes one chi^2 partial per workgroup to A.partial; the
+// number of workgroups (gemm_workgroups) is stored there, 0 when no pass
+// applied the epilogue.
 template <typename T>
 int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, const cx<T> *vin,
-                      bool shaped, hipStream_t stream);
+                      bool shaped, hipStream_t stream, uint64_t *epilogue_wgs = nullptr);
+
+// Workgroups of the matrix-core passes (one per time and channel, padded
+// to the XCD order).
+uint64_t gemm_workgroups(const mbx_dims &d);
 
 }  // namespace mbx

@@ -107,14 +107,16 @@ def test_fused_forward_many_baselines_chunked(dtype):
     assert abs(out['X2'] - ref['X2']) < TOL[dtype] * abs(ref['X2'])
 
 
-@pytest.mark.parametrize('dtype', [np.float64, np.float32])
-def test_source_batches_match_single_launch(dtype):
+@pytest.mark.parametrize('dtype,na', [(np.float64, 8), (np.float32, 8), (np.float64, 48), (np.float32, 40)])
+def test_source_batches_match_single_launch(dtype, na):
     """Reference batch semantics: lower extent > 0 reads the running sum,
-    upper extent < global writes it (RimeSumCoherencies.py:183-190, 307-316)."""
+    upper extent < global writes it (RimeSumCoherencies.py:183-190, 307-316).
+    From 40 (float) / 48 (float64) antennas the batches run through the
+    matrix-core passes, in float with the epilogue in the shaped pass."""
     import torch
     from montblanc_amd import _lib, rime
     counts = (7, 5, 3)
-    a = synth.make_inputs(8, 3, 4, *counts, beam=(9, 9, 5), seed=31, flag_fraction=0.3)
+    a = synth.make_inputs(na, 3, 4, *counts, beam=(9, 9, 5), seed=31, flag_fraction=0.3)
     arrays = rime.to_device(synth.as_dtype(a, dtype))
     whole = rime.forward(arrays, counts)
     torch.cuda.synchronize()

@@ -73,3 +73,27 @@ def test_shaped_matrix_pass_many_sources(na):
     counts = (3, 37, 29)
     a = synth.make_inputs(na, 2, 2, *counts, beam=(9, 8, 6), seed=na, flag_fraction=0.1)
     check(a, counts, np.float32)
+
+
+@pytest.mark.parametrize('residual,use_w', [(False, False), (True, False), (False, True)])
+def test_shaped_matrix_pass_applies_epilogue(residual, use_w):
+    """In float from 40 antennas the shaped matrix pass applies the
+    epilogue itself: gains, flags, residual mode, weights, chi^2 terms and
+    their sum, with baselines in either antenna order (the baseline's own
+    order picks the gains) and auto-correlations."""
+    from test_gpu_forward import check_outputs, run_forward
+    counts = (5, 6, 4)
+    a = synth.make_inputs(40, 2, 3, *counts, beam=(9, 8, 6), seed=77, flag_fraction=0.3,
+                          auto_correlations=True, observed='model')
+    a1, a2 = a['antenna1'].copy(), a['antenna2'].copy()
+    flip = np.zeros(a1.shape[1], bool)
+    flip[::3] = True
+    a['antenna1'] = np.where(flip, a2, a1).astype(a1.dtype)
+    a['antenna2'] = np.where(flip, a1, a2).astype(a2.dtype)
+    if use_w:
+        a['weight_vector'] = np.random.default_rng(3).uniform(0.5, 2.0, a['weight_vector'].shape)
+    a['sigma_sqrd'] = 1.7
+    ref = rime_np.rime(synth.upcast(synth.as_dtype(a, np.float32)), *counts, residual=residual,
+                       use_weight_vector=use_w, sigma_sqrd=1.7)
+    out = run_forward(a, counts, np.float32, residual=residual, use_weight_vector=use_w)
+    check_outputs(out, ref, a, np.float32, residual, use_w, 1.7)
