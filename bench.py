@@ -375,8 +375,8 @@ def main():
         'roofline': {'bound': 'mfma', 'achieved': r['tflops'], 'peak': peak,
                      'unit': 'TFLOP/s', 'frac': r['tflops'] / peak,
                      'traffic': measured_traffic(args.workload, primary),
-                     'kernel': 'mbx_forward_%s (k_beam_planes, k_point_gemm, k_shaped_mx, '
-                               'k_forward_pc epilogue, k_chi2_final)' % primary,
+                     'kernel': 'mbx_forward_%s (k_beam_planes, k_point_gemm, k_shaped_mx with '
+                               'the epilogue, k_chi2_final)' % primary,
                      'flops_per_launch': r['flops'],
                      'peak_note': 'compute-bound: point sources on the matrix cores (16x16x4 '
                                   'products), gaussian and sersic sources on the matrix cores '
