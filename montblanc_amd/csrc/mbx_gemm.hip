@@ -91,6 +91,17 @@ template <> __device__ __forceinline__ double shape_f<double>(bool gauss, double
     return 1.0 / (den * sqrt(den));
 }
 
+// The value of lane l ^ 1 (DPP quad permutation 1, 0, 3, 2; no LDS).
+__device__ __forceinline__ float swap_lane_pairs(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double swap_lane_pairs(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 // SHAPED = false: the point sources [0, npsrc); true: the gaussian and
 // sersic sources [npsrc, nsrc). Separate instances keep each one's
 // registers to what it needs.
@@ -221,11 +232,15 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
                             const int q = min(cb >> 1, NA - 1);
                             const T xq = XYs[2 * q], yq = XYs[2 * q + 1];
                             // this lane's rows are antennas p0 = 8 R + 2 g and p0 + 1
-                            // (rows past 2 na: any finite factor, discarded)
+                            // (rows past 2 na: any finite factor, discarded). Its
+                            // neighbour (lane ^ 1, the other column of antenna q)
+                            // needs the same two factors: each evaluates one and
+                            // takes the other across.
                             const int p0 = 8 * tpR[j] + 2 * gv;
-                            const int pa0 = min(p0, NA - 1), pa1 = min(p0 + 1, NA - 1);
-                            const T f0 = shape_f<T>(gauss, xq - XYs[2 * pa0], yq - XYs[2 * pa0 + 1]);
-                            const T f1 = shape_f<T>(gauss, xq - XYs[2 * pa1], yq - XYs[2 * pa1 + 1]);
+                            const int pm = min(p0 + (lv & 1), NA - 1);
+                            const T fm = shape_f<T>(gauss, xq - XYs[2 * pm], yq - XYs[2 * pm + 1]);
+                            const T fo = swap_lane_pairs(fm);
+                            const T f0 = (lv & 1) ? fo : fm, f1 = (lv & 1) ? fm : fo;
                             are[j][0] = fma(f0, P[0], are[j][0]); aim[j][0] = fma(f0, Q[0], aim[j][0]);
                             are[j][1] = fma(f0, P[1], are[j][1]); aim[j][1] = fma(f0, Q[1], aim[j][1]);
                             are[j][2] = fma(f1, P[2], are[j][2]); aim[j][2] = fma(f1, Q[2], aim[j][2]);
