@@ -214,7 +214,9 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
             point_gemm=True, workspace=None):
     """Fused forward on device tensors. Returns the outputs dict
     (model_vis, chi_sqrd_result, X2 [device float64 scalar], ekb_vis?).
-    point_gemm=False keeps the point sources on the vector units;
+    point_gemm=False keeps every source on the vector units (True: point
+    sources, and from 40 antennas in float / 48 in float64 the gaussian and
+    sersic sources, on the matrix cores where the layout allows);
     workspace (a Workspace) replaces the per-device one, for concurrent
     calls on several streams."""
     dev = arrays['uvw'].device
