@@ -190,8 +190,10 @@ typedef struct mbx_outputs {
  * Point sources on the matrix cores: with baseline_map (a DEVICE copy of
  * mbx_build_baseline_map's output for the same, time-invariant layout)
  * and point_gemm != 0, the point sources' coherency sums are computed as
- * a Hermitian matrix product per (time, channel) for na <= 64, and the
- * selected kernel adds the other source types. */
+ * a Hermitian matrix product per (time, channel) for na <= 64; from 40
+ * antennas (float) or 48 (float64) the gaussian and sersic sources also
+ * run on the matrix cores (in float that pass applies the epilogue too),
+ * otherwise the selected kernel adds the other source types. */
 typedef struct mbx_pair_plan {
     const int32_t *blocks;
     int32_t nblocks, pb, qb;
