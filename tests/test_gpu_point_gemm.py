@@ -64,15 +64,16 @@ def test_too_many_antennas_falls_back(dtype):
     check(a, counts, dtype)
 
 
-@pytest.mark.parametrize('na', [41, 64])
-def test_shaped_matrix_pass_many_sources(na):
-    """Gaussian and sersic sources on the matrix cores in float (40
-    antennas and up): several source tiles, the gaussian / sersic boundary
-    inside a tile, a partial last antenna quad (41 antennas), point
+@pytest.mark.parametrize('dtype,na', [(np.float32, 41), (np.float32, 64), (np.float64, 49), (np.float64, 64)])
+def test_shaped_matrix_pass_many_sources(dtype, na):
+    """Gaussian and sersic sources on the matrix cores (float from 40
+    antennas, float64 from 48): several source tiles, the gaussian / sersic
+    boundary inside a tile, a partial last antenna quad (41) or an odd
+    antenna count (49: the last lane pair shares a clamped antenna), point
     sources before them and a flagged fraction."""
     counts = (3, 37, 29)
     a = synth.make_inputs(na, 2, 2, *counts, beam=(9, 8, 6), seed=na, flag_fraction=0.1)
-    check(a, counts, np.float32)
+    check(a, counts, dtype)
 
 
 @pytest.mark.parametrize('residual,use_w', [(False, False), (True, False), (False, True)])
