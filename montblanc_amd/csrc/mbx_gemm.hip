@@ -1,4 +1,9 @@
-// Point-source coherency sums on the matrix cores (gfx950 MFMA).
+// Coherency sums on the matrix cores (gfx950 MFMA): point sources in
+// k_point_gemm<T, false>; gaussian and sersic sources in float64 in
+// k_point_gemm<T, true> (the 16x16 products scaled per element by the
+// shape factor) and in float in k_shaped_mx (4x4-block products with the
+// factor on the B operand; it also applies the epilogue), both described
+// at their kernels.
 //
 // For one time t and channel c the point sources' part of every
 // baseline's coherency sum (shape factor 1; CPUSolver.py:562-634,
