@@ -155,6 +155,25 @@ def chunks(n, size):
     return [(s, min(n, s + size)) for s in range(0, n, size)]
 
 
+# Arrays with neither a time nor a channel axis (source parameters, the
+# beam cube): sent to the device once and kept there across chunks and
+# solves, sent again only after a transfer_<name> (the reference's v5
+# caches unchanged array slices the same way, CompositeRimeSolver.py:
+# 433-593).
+STATIC_ARRAYS = tuple(n for n, shape, _dt, _d, _t, scratch in schema.ARRAYS
+                      if not scratch and 'ntime' not in shape and 'nchan' not in shape)
+INT_ARRAYS = ('antenna1', 'antenna2')
+
+
+def _device_ready(name, v):
+    """A host array in the dtype the device path takes (as rime.to_device)."""
+    if name in INT_ARRAYS:
+        return np.ascontiguousarray(v, dtype=np.int32)
+    if name == 'flag':
+        return np.ascontiguousarray(v, dtype=np.uint8)
+    return np.ascontiguousarray(v)
+
+
 class ChunkedRimeSolver(object):
     """The v4 solver interface over problems that do not fit one device.
 
@@ -164,7 +183,14 @@ class ChunkedRimeSolver(object):
     plan asks for them), writing model visibilities back to the host array
     (overwritten, or accumulated with vis_write_mode='sum') and summing
     X2 over chunks -- the host side of the reference's v5 solve loop
-    (CompositeRimeSolver.py:1027-1215), synchronous and on one device.
+    (CompositeRimeSolver.py:793-1012, 1027-1215), on one device.
+
+    The loop is pipelined: chunk k runs on the current stream while a copy
+    stream moves chunk k + 1's inputs in through pinned host buffers and
+    chunk k - 1's outputs out, so the host-device copies overlap the
+    kernels. Two chunks are then resident at once: the planner is given
+    half of the device budget. Arrays without time or channel axes stay on
+    the device between chunks and solves (STATIC_ARRAYS).
     """
 
     def __init__(self, slvr_cfg):
@@ -183,8 +209,11 @@ class ChunkedRimeSolver(object):
         self._host = _HostArrays(cfg, self.ft, self.ct)
         self.X2 = self.ft(0.0)
         self.last_plan = None
+        self.last_solve = None
         self._rime = rime
         self._torch = torch
+        self._copy = None          # copy stream, created on first solve
+        self._static = {}          # device copies of STATIC_ARRAYS
 
     def __getattr__(self, name):
         h = self.__dict__.get('_host')
@@ -208,12 +237,26 @@ class ChunkedRimeSolver(object):
     def plan(self):
         h = self._host
         dims = dict(h.dims)
-        budget = device_budget(self._cfg, self.device)
+        # two chunks in flight (see the class docstring)
+        budget = device_budget(self._cfg, self.device) // 2
         modified, T, need = plan(dims, self.ft, budget,
                                  source_batch_size=self._cfg.get(Options.SOURCE_BATCH_SIZE, 500))
         self.last_plan = dict(modified=dict(modified), chunk_dims=T, bytes=need, budget=budget)
         log.info('montblanc planner: %s per chunk (%d bytes of %d)', dict(modified), need, budget)
         return T
+
+    def _static_on_device(self):
+        """Device copies of STATIC_ARRAYS, re-sent where transfer_* changed
+        them since the last solve. Returns the names sent."""
+        torch, h = self._torch, self._host
+        sent = []
+        for n in STATIC_ARRAYS:
+            if n in self._static and n not in h.dirty:
+                continue
+            self._static[n] = torch.from_numpy(_device_ready(n, h.arrays[n])).to(self.device)
+            sent.append(n)
+        h.dirty.difference_update(STATIC_ARRAYS)
+        return sent
 
     def solve(self):
         rime, torch = self._rime, self._torch
@@ -221,29 +264,86 @@ class ChunkedRimeSolver(object):
         T = self.plan()
         nt, nc = h.dims['ntime'], h.dims['nchan']
         counts = (h.dims['npsrc'], h.dims['ngsrc'], h.dims['nssrc'])
-        x2 = 0.0
+        jobs = [(t0, t1, c0, c1) for t0, t1 in chunks(nt, T['ntime']) for c0, c1 in chunks(nc, T['nchan'])]
         model = h.arrays['model_vis']
         chi = h.arrays['chi_sqrd_result']
         if not self._sum:
             model[...] = 0
-        for t0, t1 in chunks(nt, T['ntime']):
-            for c0, c1 in chunks(nc, T['nchan']):
-                a = h.slice(t0, t1, c0, c1)
-                arrays = rime.to_device(a, self.device)
-                out = rime.forward(arrays, counts, residual=self._residual,
-                                   use_weight_vector=self._use_w,
-                                   sigma_sqrd=float(h.props['sigma_sqrd']))
-                torch.cuda.synchronize(self.device)
-                mv = out['model_vis'].cpu().numpy()
-                if self._sum:
-                    model[t0:t1, :, c0:c1] += mv
-                else:
-                    model[t0:t1, :, c0:c1] = mv
-                chi[t0:t1, :, c0:c1] = out['chi_sqrd_result'].cpu().numpy()
-                x2 += float(out['X2'].item())
-                del arrays, out
-        self.X2 = self.ft(x2)
-        h.props['X2'] = np.float64(x2)
+        dev = self.device
+        comp = torch.cuda.current_stream(dev)
+        if self._copy is None:
+            self._copy = torch.cuda.Stream(device=dev)   # default priority
+        copy = self._copy
+        sent = self._static_on_device()
+        h.dirty.clear()
+        x2 = torch.zeros((), dtype=torch.float64, device=dev)
+        sigma = float(h.props['sigma_sqrd'])
+
+        def stage(job):
+            # chunk inputs: host slice -> pinned buffer -> device, on the
+            # copy stream; an event marks them ready
+            t0, t1, c0, c1 = job
+            a = h.slice(t0, t1, c0, c1, skip=STATIC_ARRAYS)
+            arrays = dict(self._static)
+            with torch.cuda.stream(copy):
+                for k, v in a.items():
+                    if not isinstance(v, np.ndarray):
+                        arrays[k] = v
+                        continue
+                    src = torch.from_numpy(_device_ready(k, v))
+                    pin = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
+                    pin.copy_(src)
+                    d = torch.empty(src.shape, dtype=src.dtype, device=dev)
+                    d.copy_(pin, non_blocking=True)
+                    d.record_stream(comp)
+                    arrays[k] = d
+                ready = torch.cuda.Event()
+                ready.record(copy)
+            return arrays, ready
+
+        def drain(p):
+            # a finished chunk's outputs into the host arrays
+            (t0, t1, c0, c1), mv, cs, done = p
+            done.synchronize()
+            if self._sum:
+                model[t0:t1, :, c0:c1] += mv.numpy()
+            else:
+                model[t0:t1, :, c0:c1] = mv.numpy()
+            chi[t0:t1, :, c0:c1] = cs.numpy()
+
+        pending = None
+        nxt = stage(jobs[0]) if jobs else None
+        for k, job in enumerate(jobs):
+            arrays, ready = nxt
+            comp.wait_event(ready)
+            out = rime.forward(arrays, counts, residual=self._residual, use_weight_vector=self._use_w,
+                               sigma_sqrd=sigma)
+            x2 += out['X2']
+            computed = torch.cuda.Event()
+            computed.record(comp)
+            if k + 1 < len(jobs):
+                nxt = stage(jobs[k + 1])      # overlaps this chunk's kernels
+            with torch.cuda.stream(copy):
+                copy.wait_event(computed)
+                mv = torch.empty(out['model_vis'].shape, dtype=out['model_vis'].dtype, pin_memory=True)
+                cs = torch.empty(out['chi_sqrd_result'].shape, dtype=out['chi_sqrd_result'].dtype,
+                                 pin_memory=True)
+                mv.copy_(out['model_vis'], non_blocking=True)
+                cs.copy_(out['chi_sqrd_result'], non_blocking=True)
+                out['model_vis'].record_stream(copy)
+                out['chi_sqrd_result'].record_stream(copy)
+                done = torch.cuda.Event()
+                done.record(copy)
+            if pending is not None:
+                drain(pending)
+            pending = (job, mv, cs, done)
+            del arrays, out
+        if pending is not None:
+            drain(pending)
+        total = float(x2.item())
+        self.X2 = self.ft(total)
+        h.props['X2'] = np.float64(total)
+        self.last_solve = dict(chunks=len(jobs), static_sent=sent)
         return False
 
     def initialise(self):
@@ -277,6 +377,7 @@ class _HostArrays(object):
         self.ft, self.ct = ft, ct
         self.props = {n: np.float64(v) for n, _t, v in schema.PROPERTIES}
         self.arrays = {}
+        self.dirty = set()          # arrays changed since the last solve
         a1, a2 = np.triu_indices(na, 0 if autocor else 1)
         nbl = self.dims['nbl']
         for name, shape, dtype, default, _test, scratch in schema.ARRAYS:
@@ -304,6 +405,7 @@ class _HostArrays(object):
             raise ValueError('transfer_%s: dtype %s does not match %s'
                              % (name, ary.dtype, self.arrays[name].dtype))
         self.arrays[name][...] = ary
+        self.dirty.add(name)
 
     def dim_local_size(self, *names):
         v = tuple(self.dims[n] for n in names)
@@ -311,12 +413,13 @@ class _HostArrays(object):
 
     dim_global_size = dim_local_size
 
-    def slice(self, t0, t1, c0, c1):
-        """The chunk's arrays (time slice t0:t1, channel slice c0:c1)."""
+    def slice(self, t0, t1, c0, c1, skip=()):
+        """The chunk's arrays (time slice t0:t1, channel slice c0:c1),
+        leaving out the names in ``skip``."""
         A = self.arrays
         out = {}
         for name, arr in A.items():
-            if name in ('model_vis', 'chi_sqrd_result'):
+            if name in ('model_vis', 'chi_sqrd_result') or name in skip:
                 continue
             shape = dict((n, s) for n, s, *_ in ((x[0], x[1]) for x in schema.ARRAYS))[name]
             idx = []

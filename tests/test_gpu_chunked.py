@@ -82,3 +82,31 @@ def test_sum_write_mode_accumulates():
     for _ in range(4):
         s.solve()
     np.testing.assert_allclose(s.retrieve_model_vis(), 5 * once, rtol=1e-12, atol=1e-12)
+
+
+def test_pipelined_chunks_keep_unchanged_arrays_on_device():
+    """The chunk loop (inputs of the next chunk copied in while a chunk
+    computes) sends the arrays without time or channel axes once; a second
+    solve sends none of them and gives the same result, and after
+    transfer_lm only lm is sent again, with the result following the new
+    positions (the oracle on the new inputs)."""
+    counts = (4, 2, 1)
+    a = synth.make_inputs(9, 6, 8, *counts, beam=(9, 8, 6), seed=41, flag_fraction=0.2)
+    s = configure('v5', 'double', counts, budget=300 * 1024)
+    load(s, a, np.float64)
+    s.solve()
+    first = s.retrieve_model_vis().copy()
+    print('solve 1', s.last_solve, s.last_plan['chunk_dims'], flush=True)
+    assert s.last_solve['chunks'] >= 3
+    assert sorted(s.last_solve['static_sent']) == ['E_beam', 'gauss_shape', 'lm', 'sersic_shape']
+    s.solve()
+    assert s.last_solve['static_sent'] == []
+    np.testing.assert_array_equal(s.retrieve_model_vis(), first)
+    b = dict(a)
+    b['lm'] = a['lm'] + 1e-3
+    s.transfer_lm(b['lm'].astype(np.float64))
+    s.solve()
+    assert s.last_solve['static_sent'] == ['lm']
+    ref = rime_np.rime(b, *counts)
+    assert np.abs(s.retrieve_model_vis() - ref['model_vis']).max() <= 1e-10 * np.abs(ref['model_vis']).max()
+    assert abs(float(s.X2) - ref['X2']) <= 1e-10 * ref['X2']
