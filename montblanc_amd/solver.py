@@ -330,9 +330,11 @@ class RimeSolver(object):
         supplied = supplied or {}
         source = self._slvr_cfg.get(Options.DATA_SOURCE, Options.DATA_SOURCE_DEFAULT)
         if source == Options.DATA_SOURCE_MS:
-            raise NotImplementedError('MeasurementSet loading is not part of the HIP '
-                                      'hot-path build; use data_source default/test/empty '
-                                      'and transfer_<array>() the data')
+            raise NotImplementedError('data_source ms: build the solver with data_source '
+                                      'default/test/empty and load the MeasurementSet with '
+                                      'montblanc_amd.ms_loader.MeasurementSetLoader(msfile).load('
+                                      'solver, cfg) (needs python-casacore), or transfer_<array>() '
+                                      'the columns turned into arrays by ms_loader.ms_arrays()')
         for name, a in self._arrays.items():
             if name in ignore:
                 continue
