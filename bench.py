@@ -11,10 +11,13 @@ correlations).
 
 Both precisions are timed; ``value`` is the float32 run (float64 phase
 argument and beam coordinates), the float64 run is reported beside it.
-The roofline object is for the dominant kernel (the fused forward kernel):
-algorithmic FLOPs per launch over its HIP-event-timed launch duration,
-against the MI355X vector peak. cpu_baseline times the numpy restatement of
-the reference's CPU chain on a bounded slice of the same workload.
+The roofline object is for the forward launch (its matrix-core and
+vector kernels): algorithmic FLOPs per launch over its HIP-event-timed
+duration, against the dtype's dense MI355X peak. cpu_baseline times the
+numpy restatement of the reference's CPU chain on a bounded slice of the
+same workload in one process; cpu_baseline_parallel the same slice with
+the coherency sum split over source ranges in one process per core (up to
+16).
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 """
@@ -312,22 +315,39 @@ def time_batched(steps=2):
 
 def cpu_baseline(w):
     """numpy restatement of the reference CPU chain on four time steps and
-    16 channels, all sources and baselines (~10-30 s on one core)."""
+    16 channels, all sources and baselines (~10-30 s on one core), then the
+    same slice over one worker process per core (up to 16). Returns the
+    single-process and the multi-process baseline."""
     import numpy as np
-    from oracle import rime_np, synth
+    from oracle import parallel, rime_np, synth
     sample = {k: v for k, v in w.items() if k != 'name'}
     sample.update(ntime=4, nchan=16)
     a = synth.make_inputs(**sample, seed=1001)
+    counts = (w['npsrc'], w['ngsrc'], w['nssrc'])
     t0 = time.perf_counter()
-    rime_np.rime(a, w['npsrc'], w['ngsrc'], w['nssrc'], src_chunk=64)
+    ref = rime_np.rime(a, *counts, src_chunk=64)
     dt = time.perf_counter() - t0
     nbl = w['na'] * (w['na'] - 1) // 2
-    n = (w['npsrc'] + w['ngsrc'] + w['nssrc']) * 4 * nbl * 16
-    return dict(value=n / dt, unit='source-vis terms/s', cores=1, kind='port',
-                sample='%s with ntime=4, nchan=16 (all %d sources, %d baselines): %.3g terms '
-                       'in %.1f s, numpy float64, one process; host %s, os.cpu_count()=%d'
-                       % (w.get('name', 'workload'), w['npsrc'] + w['ngsrc'] + w['nssrc'], nbl,
-                          n, dt, cpu_model(), os.cpu_count() or 0))
+    n = sum(counts) * 4 * nbl * 16
+    what = '%s with ntime=4, nchan=16 (all %d sources, %d baselines): %.3g terms' % (
+        w.get('name', 'workload'), sum(counts), nbl, n)
+    single = dict(value=n / dt, unit='source-vis terms/s', cores=1, kind='port',
+                  sample='%s in %.1f s, numpy float64, one process; host %s, os.cpu_count()=%d'
+                         % (what, dt, cpu_model(), os.cpu_count() or 0))
+    procs = max(1, min(16, os.cpu_count() or 1))
+    pr = parallel.ParallelRime(a, *counts, processes=procs)
+    try:
+        t0 = time.perf_counter()
+        out = pr()
+        dtp = time.perf_counter() - t0
+    finally:
+        pr.close()
+    diff = float(np.abs(out['model_vis'] - ref['model_vis']).max() / np.abs(ref['model_vis']).max())
+    multi = dict(value=n / dtp, unit='source-vis terms/s', cores=procs, kind='port',
+                 sample='%s in %.2f s, numpy float64, %d worker processes (source ranges; pool '
+                        'started before timing); model vs one process: max rel diff %.1e'
+                        % (what, dtp, procs, diff))
+    return single, multi
 
 
 def main():
@@ -413,7 +433,7 @@ def main():
         line['large'] = time_large()
         print('# large: %s' % line['large'], file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        line['cpu_baseline'] = cpu_baseline(w)
+        line['cpu_baseline'], line['cpu_baseline_parallel'] = cpu_baseline(w)
     print(json.dumps(line), flush=True)
 
 

@@ -381,3 +381,21 @@ def test_planted_mistakes_are_caught(monkeypatch):
     test_b_sqrt_matches_scipy_sqrtm_and_invariants()
     test_rime_brute_force(False, False)
     test_point_source_fourier_relation()
+
+
+def test_parallel_oracle_matches_single_process():
+    """The multi-core CPU baseline (oracle/parallel.py): source ranges over
+    two worker processes, including ranges that straddle the type
+    boundaries, give the single-process oracle's outputs."""
+    from oracle import parallel
+    counts = (5, 4, 3)
+    a = synth.make_inputs(6, 2, 3, *counts, beam=(9, 8, 6), seed=3, flag_fraction=0.2)
+    assert parallel.source_ranges(12, 5) == [(0, 2), (2, 5), (5, 7), (7, 10), (10, 12)]
+    ref = rime_np.rime(a, *counts)
+    p = parallel.ParallelRime(a, *counts, processes=2)
+    try:
+        out = p()
+    finally:
+        p.close()
+    np.testing.assert_allclose(out['model_vis'], ref['model_vis'], rtol=1e-12, atol=1e-12)
+    assert abs(out['X2'] - ref['X2']) <= 1e-12 * ref['X2']
