@@ -98,3 +98,12 @@ def test_shaped_matrix_pass_applies_epilogue(residual, use_w):
                        use_weight_vector=use_w, sigma_sqrd=1.7)
     out = run_forward(a, counts, np.float32, residual=residual, use_weight_vector=use_w)
     check_outputs(out, ref, a, np.float32, residual, use_w, 1.7)
+
+
+@pytest.mark.parametrize('dtype,na', [(np.float32, 44), (np.float64, 48)])
+def test_shaped_matrix_pass_without_point_sources(dtype, na):
+    """No point sources: the point pass is skipped and the shaped matrix
+    pass starts from no running sum (and in float applies the epilogue)."""
+    counts = (0, 6, 4)
+    a = synth.make_inputs(na, 2, 3, *counts, beam=(9, 8, 6), seed=na + 1, flag_fraction=0.2)
+    check(a, counts, dtype)
