@@ -313,6 +313,31 @@ def time_batched(steps=2):
     return out
 
 
+def attainable_peaks(reps=5):
+    """Dense matrix-core rates a vendor library reaches on this device:
+    torch.mm (hipBLASLt / rocBLAS) of square float32 (8192) and float64
+    (4096) matrices, best of ``reps`` HIP-event-timed calls. A reference
+    point beside the datasheet peaks the roofline is priced against."""
+    import torch
+    out = {}
+    for name, dt, n in (('f32', torch.float32, 8192), ('f64', torch.float64, 4096)):
+        a = torch.randn(n, n, dtype=dt, device='cuda')
+        b = torch.randn(n, n, dtype=dt, device='cuda')
+        c = torch.mm(a, b)
+        best = float('inf')
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            torch.mm(a, b, out=c)
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        out[name + '_gemm_tflops'] = 2.0 * n ** 3 / best / 1e12
+        del a, b, c
+    out['note'] = 'torch.mm of square matrices (f32 8192, f64 4096), best of %d' % reps
+    return out
+
+
 def cpu_baseline(w):
     """numpy restatement of the reference CPU chain on four time steps and
     16 channels, all sources and baselines (~10-30 s on one core), then the
@@ -432,6 +457,10 @@ def main():
     if args.large:
         line['large'] = time_large()
         print('# large: %s' % line['large'], file=sys.stderr, flush=True)
+    att = attainable_peaks()
+    if primary + '_gemm_tflops' in att:
+        att['frac_of_' + primary + '_gemm'] = r['tflops'] / att[primary + '_gemm_tflops']
+    line['attainable'] = att
     if not args.no_cpu_baseline:
         line['cpu_baseline'], line['cpu_baseline_parallel'] = cpu_baseline(w)
     print(json.dumps(line), flush=True)
