@@ -20,11 +20,13 @@ table's rows are taken ordered by (TIME, ANTENNA1, ANTENNA2, band), field
   broadcast over the band's channels, else ones (init_weights 'weight'),
   or the same from SIGMA_SPECTRUM / SIGMA ('sigma').
 
+* parallactic_angles from the ANTENNA POSITIONs, the FIELD's PHASE_DIR
+  and the unique TIMEs (montblanc_amd/parallactic.py; the reference uses
+  casacore measures), when those columns are given.
+
 Reading the tables needs python-casacore, which ``read_ms`` imports and
 reports clearly when absent; ``ms_dims`` and ``ms_arrays`` work on column
-arrays however they were read. Parallactic angles (computed by the
-reference from antenna positions with casacore measures) are not derived
-here: the solver's parallactic_angles stay as they are.
+arrays however they were read.
 """
 
 import numpy as np
@@ -61,9 +63,11 @@ def read_ms(msfile, auto_correlations=False):
     cols = {c: main.getcol(c) for c in MAIN_COLUMNS if c in main.colnames()}
     spw = pt.table('::'.join((msfile, 'SPECTRAL_WINDOW')), ack=False, readonly=True)
     ant = pt.table('::'.join((msfile, 'ANTENNA')), ack=False, readonly=True)
+    field = pt.table('::'.join((msfile, 'FIELD')), ack=False, readonly=True)
     cols.update(CHAN_FREQ=spw.getcol('CHAN_FREQ'), REF_FREQUENCY=spw.getcol('REF_FREQUENCY'),
-                NUM_CHAN=spw.getcol('NUM_CHAN'), NA=ant.nrows())
-    for t in (main, spw, ant, ms):
+                NUM_CHAN=spw.getcol('NUM_CHAN'), NA=ant.nrows(), POSITION=ant.getcol('POSITION'),
+                PHASE_DIR=field.getcol('PHASE_DIR')[0][0])
+    for t in (main, spw, ant, field, ms):
         t.close()
     return cols
 
@@ -109,6 +113,11 @@ def ms_arrays(cols, ft=np.float64, init_weights=None):
     out['uvw'] = uvw
     out['antenna1'] = a1
     out['antenna2'] = a2
+    if 'POSITION' in cols and 'PHASE_DIR' in cols:
+        from .parallactic import parallactic_angles
+        times = np.unique(np.asarray(cols['TIME'], np.float64))
+        out['parallactic_angles'] = parallactic_angles(np.asarray(cols['PHASE_DIR'], np.float64),
+                                                       cols['POSITION'], times).astype(ft)
     if 'DATA' in cols:
         out['observed_vis'] = np.asarray(cols['DATA']).reshape(ntime, nbl, nchan, 4).astype(ct)
     if 'FLAG' in cols:

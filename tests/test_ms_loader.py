@@ -65,6 +65,18 @@ def test_columns_to_solver_arrays(auto, reverse):
     np.testing.assert_allclose(s['weight_vector'][0, 0, 0], cols['SIGMA'][0], rtol=1e-6)
 
 
+def test_parallactic_angles_from_positions():
+    from montblanc_amd import parallactic as pa
+    cols, _, _, (ntime, nbl, na, nbands, nchan) = make_columns()
+    pos = pa.itrf(np.radians(-30.7 + 0.01 * np.arange(na)), np.radians(21.4 + 0.01 * np.arange(na)), 1000.0)
+    cols = dict(cols, POSITION=pos, PHASE_DIR=np.array([1.2, -0.5]), TIME=cols['TIME'] * 0 + 4.9e9 +
+                np.repeat(np.arange(ntime) * 300.0, nbl * nbands))
+    a = ms_loader.ms_arrays(cols, np.float64)
+    want = pa.parallactic_angles(np.array([1.2, -0.5]), pos, 4.9e9 + np.arange(ntime) * 300.0)
+    assert a['parallactic_angles'].shape == (ntime, na)
+    np.testing.assert_allclose(a['parallactic_angles'], want, atol=1e-15)
+
+
 def test_layout_checks_and_missing_casacore():
     cols, _, _, _ = make_columns()
     bad = dict(cols, NUM_CHAN=np.array([3, 4]))
