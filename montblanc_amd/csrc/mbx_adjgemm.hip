@@ -44,14 +44,37 @@ static __host__ __device__ inline size_t adjg_setup_bytes(int na) {
     return ((sizeof(double) * (9 * size_t(na) + CH)) + 31) & ~size_t(31);
 }
 
+// Shaped pass record strides (elements). W block (p, q) sits at
+// p WPS + 8 q, X record (s, a) at s XPS + 10 a; the row padding puts the
+// rows of the eight antennas (W) or sources (X) one wave reads at once in
+// different LDS banks (WPS / 4 and XPS / 2 odd).
+__host__ __device__ inline int adjs_wps(int na) { return 8 * na + 4; }
+__host__ __device__ inline int adjs_xps(int na) { return 10 * na + ((na & 1) ? 4 : 2); }
+
 template <typename T>
-static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw) {
-    return adjg_setup_bytes(na) + 2 * size_t(rows) * rows * sizeof(T)   // W planes
-           + 2 * size_t(2 * st) * rows * sizeof(T)                      // X planes (then Jb)
+static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, bool shaped) {
+    const size_t w = shaped ? size_t(na) * adjs_wps(na) : 2 * size_t(rows) * rows;   // W
+    const size_t x = shaped ? size_t(st) * adjs_xps(na) : 2 * size_t(2 * st) * rows; // X (then Jb)
+    return adjg_setup_bytes(na) + (w + x) * sizeof(T)
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
            + size_t(st) * 4 * sizeof(double)                            // l, m, n - 1
            + size_t(st) * 24 * sizeof(T)                                // shape, I Q U V P, sqrt(B)
-           + (2 * size_t(na) * st + 2 * size_t(na)) * sizeof(T);        // shape coordinates, u, v
+           + (shaped ? 2 * size_t(na) : 0) * sizeof(T);                 // u, v
+}
+
+// A Jones matrix from eight consecutive values (re, im interleaved) at an
+// address aligned to two values.
+template <typename T>
+__device__ __forceinline__ J4<T> jload_pairs(const T *p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 *q = reinterpret_cast<const v2 *>(p);
+    J4<T> r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const v2 a = q[i];
+        r.v[i] = {a.x, a.y};
+    }
+    return r;
 }
 
 // Sersic factor of a pair from r2 = dx^2 + dy^2: (1 + r2)^(-3/2), with
@@ -100,45 +123,57 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     double *Lpa = Luvw + 3 * NA;
     double *Lant = Lpa + 2 * NA;
     double *Lch = Lant + 4 * NA;
+    // Point pass: W and X as separate re and im planes; X is [n][row] (MFMA
+    // operand reads along the rows), n = 2 s + j, row = 2 a + k.
+    // Shaped pass: complex interleaved records, walked by pointers over q:
+    //   W block (p, q) at Wr + p WPS + 8 q: W_pq[i][k] at 2 (2 i + k) (re, im);
+    //   X record (s, a) at Xr + s XPS + 10 a: J_as[k][j] at 2 (2 k + j)
+    //   (re, im), then the shape coordinates x, y of (antenna a, source s).
+    const int WPS = adjs_wps(NA), XPS = adjs_xps(NA);
+    const int wsize = SHAPED ? NA * WPS : 2 * rows * rows;
+    const int xsize = SHAPED ? st * XPS : 4 * st * rows;
     T *Wr = reinterpret_cast<T *>(smem_raw + adjg_setup_bytes(NA));
     T *Wi = Wr + rows * rows;
-    // X (re, im planes): point pass [n][row] (MFMA operand reads along the
-    // rows), shaped pass [row][n] (a wave reads one row for many sources);
-    // n = 2 s + j, row = 2 a + k
-    T *Xr = Wi + rows * rows;
+    T *Xr = Wr + wsize;
     T *Xi = Xr + 2 * st * rows;
     // Point pass operands are blocked by four along the summed index, so a
     // lane's four consecutive k values are one 16/32-byte read and
     // consecutive lanes (rows of W, sources of X) read consecutive blocks:
     //   W[row][col] at ((col / 4) rows + row) 4 + col % 4,
     //   X[n][row]   at ((row / 4) 2 st + n) 4 + row % 4   (point pass)
-    auto xi_ = [&](int n, int row) {
-        return SHAPED ? row * (2 * st) + n : ((row >> 2) * (2 * st) + n) * 4 + (row & 3);
-    };
+    auto xi_ = [&](int n, int row) { return ((row >> 2) * (2 * st) + n) * 4 + (row & 3); };
     auto wi_ = [&](int row, int col) { return ((col >> 2) * rows + row) * 4 + (col & 3); };
     J4<T> *Jb = reinterpret_cast<J4<T> *>(Xr); // [st][NA], written over X
-    double *sacc = reinterpret_cast<double *>(Xi + 2 * st * rows);   // [st][10]
+    double *sacc = reinterpret_cast<double *>(Xr + xsize);          // [st][10]
     double *Lsd = sacc + S.sw * 10;                                 // [st][4]: l, m, n - 1
     T *Lsh = reinterpret_cast<T *>(Lsd + st * 4);                   // [st][8]: shape record
     T *Lst = Lsh + st * 8;                                          // [st][16]: IQUV P . sqrt(B)
-    // shaped pass: shape coordinates x, y of (antenna, source) [2][na][st]
-    // and the antennas' u, v [2][na]
-    T *Lxy = Lst + st * 16;
-    T *Luv = Lxy + 2 * NA * st;
+    T *Luv = Lst + st * 16;                                         // shaped: (u, v) per antenna
     // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na],
     // written over X once every X read is done (they fit: (kSplit - 1) st na
-    // J4 <= 2 planes x 2 st x rows for kSplit = 2, since 2 na <= rows)
+    // J4 <= st XPS for kSplit = 2, since XPS > 8 na)
     static_assert(kSplit <= 2, "the partials are kept over X");
     J4<T> *Jpart = Jb;
 
     tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kThreads);
     stage_antennas<T>(R, Lant, t, c, tid, kThreads);
-    for (int i = tid; i < 2 * rows * rows; i += kThreads) Wr[i] = T(0);
+    for (int i = tid; i < wsize; i += kThreads) Wr[i] = T(0);
     for (int i = tid; i < S.sw * 10; i += kThreads) sacc[i] = 0.0;
     __syncthreads();
     if (SHAPED)
-        for (int a = tid; a < NA; a += kThreads) { Luv[a] = T(Luvw[3 * a]); Luv[NA + a] = T(Luvw[3 * a + 1]); }
+        for (int a = tid; a < NA; a += kThreads) { Luv[2 * a] = T(Luvw[3 * a]); Luv[2 * a + 1] = T(Luvw[3 * a + 1]); }
     // W from this (t, c)'s visibility adjoints
+    // (re, im) of W_pq[i][k]: point pass planes, shaped pass records
+    auto wput = [&](int p, int i, int q, int k, T re, T im) {
+        if (SHAPED) {
+            T *w = Wr + p * WPS + 8 * q + 2 * (2 * i + k);
+            w[0] = re;
+            w[1] = im;
+        } else {
+            Wr[wi_(2 * p + i, 2 * q + k)] = re;
+            Wi[wi_(2 * p + i, 2 * q + k)] = im;
+        }
+    };
     for (int b = tid; b < nb_ext; b += kThreads) {
         const int p = min(max(R.a1[uint64_t(t) * NBL + b], 0), NA - 1);
         const int q = min(max(R.a2[uint64_t(t) * NBL + b], 0), NA - 1);
@@ -150,13 +185,10 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 const cx<T> v = V.v[2 * i + k];
                 if (p == q) {
                     const cx<T> u = V.v[2 * k + i];   // + (Vb^H)[i][k] = conj(Vb[k][i])
-                    Wr[wi_(2 * p + i, 2 * p + k)] = v.re + u.re;
-                    Wi[wi_(2 * p + i, 2 * p + k)] = v.im - u.im;
+                    wput(p, i, p, k, v.re + u.re, v.im - u.im);
                 } else {
-                    Wr[wi_(2 * p + i, 2 * q + k)] = v.re;
-                    Wi[wi_(2 * p + i, 2 * q + k)] = v.im;
-                    Wr[wi_(2 * q + k, 2 * p + i)] = v.re;
-                    Wi[wi_(2 * q + k, 2 * p + i)] = -v.im;
+                    wput(p, i, q, k, v.re, v.im);
+                    wput(q, k, p, i, v.re, -v.im);
                 }
             }
     }
@@ -225,13 +257,22 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 J = item_forward<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[0], an[1], an[2],
                                     an[3], c, Lch[3], Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], o[0], o[1],
                                     o[2], o[3], o[4], ist);
+            T *xrec = Xr + isi * XPS + 10 * ia;
+            if (SHAPED) {
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    Xr[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].re;
-                    Xi[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].im;
+                for (int x = 0; x < 4; ++x) {
+                    xrec[2 * x] = J.v[x].re;
+                    xrec[2 * x + 1] = J.v[x].im;
                 }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        Xr[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].re;
+                        Xi[xi_(2 * isi + j, 2 * ia + k)] = J.v[2 * k + j].im;
+                    }
+            }
             if (SHAPED) {
                 // shape coordinates, linear in the antenna's (u, v):
                 // f_pq = F((x_q - x_p)^2 + (y_q - y_p)^2)
@@ -246,13 +287,13 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     x = (u * (1.0 + e1) + v * e2) * double(h[1]);
                     y = (u * e2 + v * (1.0 - e1)) * double(h[1]);
                 }
-                Lxy[ia * st + isi] = T(x);
-                Lxy[(NA + ia) * st + isi] = T(y);
+                xrec[8] = T(x);
+                xrec[9] = T(y);
             }
         }
         for (int i = tid; i < (SHAPED ? 0 : 2 * st * rows); i += kThreads) {
-            const int n = SHAPED ? i % (2 * st) : (i >> 2) % (2 * st);
-            const int r = SHAPED ? i / (2 * st) : (i / (8 * st)) * 4 + (i & 3);
+            const int n = (i >> 2) % (2 * st);
+            const int r = (i / (8 * st)) * 4 + (i & 3);
             if (n >= 2 * nst || r >= 2 * NA) { Xr[i] = T(0); Xi[i] = T(0); }
         }
         __syncthreads();
@@ -323,37 +364,25 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             J4<T> acc = jzero<T>();
             T sh0 = T(0), sh1 = T(0), sh2 = T(0);
             if (mine) {
+                typedef T v2 __attribute__((ext_vector_type(2)));
                 const T *h = Lsh + 8 * si;
-                J4<T> Jp;
-#pragma unroll
-                for (int k = 0; k < 2; ++k)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        Jp.v[2 * k + j] = {Xr[xi_(2 * si + j, 2 * p + k)], Xi[xi_(2 * si + j, 2 * p + k)]};
-                const T *Lx = Lxy + si, *Ly = Lxy + NA * st + si;
-                const T xp = Lx[p * st], yp = Ly[p * st], up = Luv[p], vp = Luv[NA + p];
-                // running LDS offsets of pair q: W rows 2p, 2p + 1 at columns
-                // 2q, 2q + 1 (wo + 4 i + k), X rows 2q, 2q + 1 of source si
-                // (xo + 2 st i + k), shape coordinates (lo)
-                uint32_t wo = (uint32_t(q0 >> 1) * rows + 2 * p) * 4 + 2 * (q0 & 1);
-                uint32_t xo = uint32_t(2 * q0) * (2 * st) + 2 * si;
-                uint32_t lo = uint32_t(q0) * st;
-                const uint32_t xs2 = 2 * st;
-                auto pair_y = [&](int q) {
-                    J4<T> Wq, Jq;
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int k = 0; k < 2; ++k) {
-                            Wq.v[2 * i + k] = {Wr[wo + 4 * i + k], Wi[wo + 4 * i + k]};
-                            Jq.v[2 * i + k] = {Xr[xo + xs2 * i + k], Xi[xo + xs2 * i + k]};
-                        }
-                    return jmul(Wq, Jq);
+                const T *xs = Xr + si * XPS;            // this source's X records
+                const J4<T> Jp = jload_pairs(xs + 10 * p);
+                const v2 xyp = *reinterpret_cast<const v2 *>(xs + 10 * p + 8);
+                const v2 uvp = *reinterpret_cast<const v2 *>(Luv + 2 * p);
+                const T xp = xyp.x, yp = xyp.y, up = uvp.x, vp = uvp.y;
+                // pair q's W block, X record and (u, v): one running pointer
+                // each, fixed offsets within a record
+                const T *wq = Wr + p * WPS + 8 * q0;
+                const T *xq = xs + 10 * q0;
+                const T *uq = Luv + 2 * q0;
+                auto pair_y = [&](int) {
+                    return jmul(jload(reinterpret_cast<const cx<T> *>(wq)), jload_pairs(xq));
                 };
-                auto advance = [&](int q) {
-                    wo += (q & 1) ? 4 * uint32_t(rows) - 2 : 2;
-                    xo += 2 * xs2;
-                    lo += st;
+                auto advance = [&](int) {
+                    wq += 8;
+                    xq += 10;
+                    uq += 2;
                 };
                 auto accumulate = [&](T f, const J4<T> &Y) {
 #pragma unroll
@@ -369,17 +398,19 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
 #pragma unroll kQUnroll
                     for (int q = q0; q < q1; ++q) {
                         const J4<T> Y = pair_y(q);
-                        const T dx = Lx[lo] - xp, dy = Ly[lo] - yp;
-                        advance(q);
+                        const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
+                        const T dx = xy.x - xp, dy = xy.y - yp;
                         const T f = gauss_of(fma(dx, dx, dy * dy));
                         accumulate(f, Y);
                         if (q > p) {
-                            const T du = Luv[q] - up, dv = Luv[NA + q] - vp;
+                            const v2 uv = *reinterpret_cast<const v2 *>(uq);
+                            const T du = uv.x - up, dv = uv.y - vp;
                             const T tq = T(-4) * f * k * jdot_re(Jp, Y);
                             sh0 = fma(tq, dy * du - Rg * dx * dv, sh0);
                             sh1 = fma(tq, Rg * dx * du + dy * dv, sh1);
                             sh2 = fma(tq, dx * (du * em - dv * el), sh2);
                         }
+                        advance(q);
                     }
                 } else {
                     // sersic: D = 1 + r2, f = D^-3/2, c = -3 f / D; d f / d(e1,
@@ -390,20 +421,22 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
 #pragma unroll kQUnroll
                     for (int q = q0; q < q1; ++q) {
                         const J4<T> Y = pair_y(q);
-                        const T dx = Lx[lo] - xp, dy = Ly[lo] - yp;
-                        advance(q);
+                        const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
+                        const T dx = xy.x - xp, dy = xy.y - yp;
                         const T r2 = fma(dx, dx, dy * dy);
                         T id;
                         const T f = sersic_factor(r2, id);
                         accumulate(f, Y);
                         if (q > p) {
-                            const T du = Luv[q] - up, dv = Luv[NA + q] - vp;
+                            const v2 uv = *reinterpret_cast<const v2 *>(uq);
+                            const T du = uv.x - up, dv = uv.y - vp;
                             const T tq = T(-6) * f * id * jdot_re(Jp, Y);
                             sh0 = fma(tq, kR * (dx * du - dy * dv) + a1 * r2, sh0);
                             sh1 = fma(tq, kR * (dx * dv + dy * du) + a2 * r2, sh1);
                             sh2 = fma(tq, kd * (dx * (du * (T(1) + e1) + dv * e2) + dy * (du * e2 + dv * (T(1) - e1))),
                                       sh2);
                         }
+                        advance(q);
                     }
                 }
             }
@@ -792,21 +825,21 @@ static bool adjp_shape(const mbx_dims &d, AdjPointShape &S, size_t &lds) {
 }
 
 template <typename T>
-static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds, int threads = kAdjThreads,
-                       int split = 1) {
+static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds, bool shaped,
+                       int threads = kAdjThreads, int split = 1) {
     const int na = d.na.local_size;
     if (na <= 0 || na > 64) return false;
     S.rows = ((2 * na + 15) / 16) * 16;
     S.st = std::min(16, (threads / split / na) / 8 * 8);
     if (S.st < 8) return false;
     static size_t budget = gemm_lds_budget();
-    auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw) <= budget; };
+    auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw, shaped) <= budget; };
     if (!fits(S.st, S.st) && S.st > 8) S.st = 8;
     if (!fits(S.st, S.st)) return false;
     // the widest flush window that fits, up to 32 tiles
     S.sw = S.st;
     while (S.sw < 32 * S.st && fits(S.st, S.sw + S.st)) S.sw += S.st;
-    lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw);
+    lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw, shaped);
     return (S.rows / 16) * (S.st / 8) <= kAdjMfmaWaves * kAdjMaxTiles;
 }
 
@@ -814,7 +847,7 @@ template <typename T>
 bool adjoint_gemm_supported(const mbx_dims &d) {
     AdjGemmShape S;
     size_t lds;
-    return adjg_shape<T>(d, S, lds);
+    return adjg_shape<T>(d, S, lds, false) && adjg_shape<T>(d, S, lds, true);
 }
 
 template <typename T>
@@ -822,7 +855,7 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
     const mbx_dims &d = A.R.d;
     AdjGemmShape S;
     size_t lds;
-    if (!adjg_shape<T>(d, S, lds)) return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: layout");
+    if (!adjg_shape<T>(d, S, lds, false)) return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: layout");
     const unsigned nwg = 8 * tc_per_xcd(dext(d.ntime), dext(d.nchan));
     AdjArgs<T> Ag = A;
     Ag.R.ct = 1;
@@ -846,12 +879,15 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
         bool wide = false;
         if constexpr (sizeof(T) == 4) {
             constexpr int wth = adjg_threads<T, true>(), wsp = adjg_split<T, true>();
-            wide = adjg_shape<T>(d, SW, wlds, wth, wsp);
+            wide = adjg_shape<T>(d, SW, wlds, true, wth, wsp);
             if (wide)
                 hipLaunchKernelGGL((k_adj_gemm<T, true, true>), dim3(nwg), dim3(wth), wlds, stream, Ag, SW);
         }
-        if (!wide)
-            hipLaunchKernelGGL((k_adj_gemm<T, true>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
+        if (!wide) {
+            if (!adjg_shape<T>(d, SW, wlds, true))
+                return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: shaped layout");
+            hipLaunchKernelGGL((k_adj_gemm<T, true>), dim3(nwg), dim3(kAdjThreads), wlds, stream, Ag, SW);
+        }
         return hip_status(hipGetLastError(), "adjoint shaped launch");
     }
     return MBX_OK;
