@@ -62,6 +62,10 @@ static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, bool shape
            + (shaped ? 2 * size_t(na) : 0) * sizeof(T);                 // u, v
 }
 
+// Gaussian shape coordinates are kept scaled by sqrt(log2 e) in float, so
+// that a pair's exp(-r2) is one exp2 of the scaled r2 (gauss_of_scaled).
+template <typename T> constexpr double adj_gauss_pre() { return sizeof(T) == 4 ? 1.2011224087864498 : 1.0; }
+
 // A Jones matrix from eight consecutive values (re, im interleaved) at an
 // address aligned to two values.
 template <typename T>
@@ -280,8 +284,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 const double u = Luvw[3 * ia], v = Luvw[3 * ia + 1];
                 double x, y;
                 if (h[0] == T(1)) {
-                    x = (u * double(h[4]) - v * double(h[3])) * double(h[2]);
-                    y = (u * double(h[3]) + v * double(h[4])) * double(h[1]);
+                    x = (u * double(h[4]) - v * double(h[3])) * double(h[2]) * adj_gauss_pre<T>();
+                    y = (u * double(h[3]) + v * double(h[4])) * double(h[1]) * adj_gauss_pre<T>();
                 } else {
                     const double e1 = h[5], e2 = h[6];
                     x = (u * (1.0 + e1) + v * e2) * double(h[1]);
@@ -391,33 +395,46 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         acc.v[x].im = fma(f, Y.v[x].im, acc.v[x].im);
                     }
                 };
+                // Each pair's shape-factor adjoint is t_q = f Re <J_p, W_pq J_q>
+                // (D = 1 + r2 for sersic: t_q = f Re <..> / D) times a form
+                // linear in (dx du, dx dv, dy du, dy dv) and, for sersic, r2:
+                // those moments are summed over q and combined once.
+                T m_xu = T(0), m_xv = T(0), m_yu = T(0), m_yv = T(0);
+                auto moments = [&](T tq, T dx, T dy) {
+                    const v2 uv = *reinterpret_cast<const v2 *>(uq);
+                    const T du = uv.x - up, dv = uv.y - vp;
+                    const T tx = tq * dx, ty = tq * dy;
+                    m_xu = fma(tx, du, m_xu);
+                    m_xv = fma(tx, dv, m_xv);
+                    m_yu = fma(ty, du, m_yu);
+                    m_yv = fma(ty, dv, m_yv);
+                };
                 if (h[0] == T(1)) {
                     // gaussian: f = exp(-r2); d f / d(el, em, R) = -2 f k
-                    // (dy du - R dx dv, R dx du + dy dv, dx (du em - dv el))
+                    // (dy du - R dx dv, R dx du + dy dv, dx (du em - dv el));
+                    // dx, dy here are scaled by g = adj_gauss_pre
                     const T k = h[1], el = h[3], em = h[4], Rg = h[5];
 #pragma unroll kQUnroll
                     for (int q = q0; q < q1; ++q) {
                         const J4<T> Y = pair_y(q);
                         const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
                         const T dx = xy.x - xp, dy = xy.y - yp;
-                        const T f = gauss_of(fma(dx, dx, dy * dy));
+                        const T f = gauss_of_scaled(fma(dx, dx, dy * dy));
                         accumulate(f, Y);
-                        if (q > p) {
-                            const v2 uv = *reinterpret_cast<const v2 *>(uq);
-                            const T du = uv.x - up, dv = uv.y - vp;
-                            const T tq = T(-4) * f * k * jdot_re(Jp, Y);
-                            sh0 = fma(tq, dy * du - Rg * dx * dv, sh0);
-                            sh1 = fma(tq, Rg * dx * du + dy * dv, sh1);
-                            sh2 = fma(tq, dx * (du * em - dv * el), sh2);
-                        }
+                        if (q > p) moments(f * jdot_re(Jp, Y), dx, dy);
                         advance(q);
                     }
+                    const T c4 = T(-4) * k / T(adj_gauss_pre<T>());
+                    sh0 = c4 * (m_yu - Rg * m_xv);
+                    sh1 = c4 * fma(Rg, m_xu, m_yv);
+                    sh2 = c4 * (em * m_xu - el * m_xv);
                 } else {
                     // sersic: D = 1 + r2, f = D^-3/2, c = -3 f / D; d f / d(e1,
                     // e2, R) = c (kR' (dx du - dy dv) + a1 r2, kR' (dx dv + dy du)
                     // + a2 r2, k/den (dx dx0 + dy dy0)) with dx0 = du (1 + e1)
                     // + dv e2, dy0 = du e2 + dv (1 - e1)
                     const T kR = h[1], a1 = h[2], a2 = h[3], kd = h[4], e1 = h[5], e2 = h[6];
+                    T m_r = T(0);
 #pragma unroll kQUnroll
                     for (int q = q0; q < q1; ++q) {
                         const J4<T> Y = pair_y(q);
@@ -428,16 +445,15 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         const T f = sersic_factor(r2, id);
                         accumulate(f, Y);
                         if (q > p) {
-                            const v2 uv = *reinterpret_cast<const v2 *>(uq);
-                            const T du = uv.x - up, dv = uv.y - vp;
-                            const T tq = T(-6) * f * id * jdot_re(Jp, Y);
-                            sh0 = fma(tq, kR * (dx * du - dy * dv) + a1 * r2, sh0);
-                            sh1 = fma(tq, kR * (dx * dv + dy * du) + a2 * r2, sh1);
-                            sh2 = fma(tq, kd * (dx * (du * (T(1) + e1) + dv * e2) + dy * (du * e2 + dv * (T(1) - e1))),
-                                      sh2);
+                            const T tq = f * id * jdot_re(Jp, Y);
+                            moments(tq, dx, dy);
+                            m_r = fma(tq, r2, m_r);
                         }
                         advance(q);
                     }
+                    sh0 = T(-6) * fma(kR, m_xu - m_yv, a1 * m_r);
+                    sh1 = T(-6) * fma(kR, m_xv + m_yu, a2 * m_r);
+                    sh2 = T(-6) * kd * ((T(1) + e1) * m_xu + e2 * (m_xv + m_yu) + (T(1) - e1) * m_yv);
                 }
             }
             __syncthreads();                       // every X read done
