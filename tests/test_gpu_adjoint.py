@@ -100,13 +100,16 @@ def test_adjoint_many_point_tiles(dtype, na, counts):
         assert e < tol, (name, e)
 
 
-@pytest.mark.parametrize('dtype', [np.float64, np.float32])
-def test_adjoint_many_shaped_tiles(dtype):
+@pytest.mark.parametrize('dtype,na', [(np.float64, 27), (np.float32, 27), (np.float64, 40),
+                                      (np.float32, 40), (np.float32, 64)])
+def test_adjoint_many_shaped_tiles(dtype, na):
     """More gaussian and sersic sources than one gradient window holds (up
     to 32 source tiles): the shaped pass's window flushes, and in float its
-    sixteen-wave layout with the pair loop split over two threads."""
-    counts = (4, 600, 30)
-    a = problem(27, 1, 2, counts, seed=77)
+    sixteen-wave layout with the pair loop split over two threads. Odd and
+    even antenna counts take the two record-row paddings of the shaped
+    pass's X records."""
+    counts = (4, 600, 30) if na < 64 else (2, 150, 30)
+    a = problem(na, 1, 2, counts, seed=77 + na)
     ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts, sigma_sqrd=1.3)
     got, _ = gpu_grads(a, counts, dtype, False)
     tol = TOL[dtype] if dtype == np.float64 else 3e-4
