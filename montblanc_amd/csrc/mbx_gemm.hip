@@ -448,17 +448,33 @@ __device__ __forceinline__ void mx_sources(const XRow<float> *Xp, const XRow<flo
     for (int s = s0; s < s1; ++s) {
         const XRow<float> b0 = Xq[0], b1 = Xq[1];
         const float2 cq = XYq[0];
+        // the source's scaled B operands first, each in its own register,
+        // then the products: no vector write sits right before the
+        // matrix instruction that reads it
+        XRow<float> a[SETS];
+        float fb[SETS][2][4];
 #pragma unroll
         for (int g = 0; g < SETS; ++g) {
-            const XRow<float> a = Xp[4 * g];
+            a[g] = Xp[4 * g];
             const float2 cp = XYp[g];
             const float f = shape_f<float>(GAUSS, cq.x - cp.x, cq.y - cp.y);
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                D[g][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a.v[m], f * b0.v[m], D[g][0], 0, 0, 0);
-                D[g][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a.v[m], f * b1.v[m], D[g][1], 0, 0, 0);
+                fb[g][0][m] = f * b0.v[m];
+                fb[g][1][m] = f * b1.v[m];
             }
         }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int g = 0; g < SETS; ++g) {
+                D[g][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[g].v[m], fb[g][0][m], D[g][0], 0, 0, 0);
+                D[g][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[g].v[m], fb[g][1][m], D[g][1], 0, 0, 0);
+            }
+        // schedule: every vector instruction of the source, then its
+        // 8 SETS matrix instructions
+        __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 8 * SETS, 0);
         Xp += 4 * na; Xq += 4 * na; XYp += na; XYq += na;
     }
 }
