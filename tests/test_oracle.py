@@ -208,37 +208,189 @@ def test_e_beam_degenerate_cube():
 
 
 # ---- shapes ------------------------------------------------------------
+#
+# Both shape factors are checked against what they are meant to be: the
+# normalised Fourier transform of an image-plane brightness, at the
+# baseline's spatial frequency (u, v) * f / C. The brightness is built on a
+# grid from its own definition (a Gaussian given by its full widths at half
+# maximum; an exponential disk carried onto the sky by a shear matrix) and
+# transformed by a direct sum, so the constants (gauss_scale, 2 pi / C,
+# fwhm2int), the (el, em, R) axis convention and the (e1, e2) shear
+# convention are all pinned without retyping the oracle's expression.
 
-def test_gaussian_shape_closed_form_and_limits():
+def _fourier_sum(image, x, ul, vl):
+    """Normalised direct Fourier sum of an image on the square grid x * x
+    (first axis l, second m) at spatial frequency (ul, vl) wavelengths."""
+    ph = np.exp(-2j * np.pi * ul * x)[:, None] * np.exp(-2j * np.pi * vl * x)[None, :]
+    return (image * ph).sum() / image.sum()
+
+
+def _uvw_at(ul, vl, f):
+    """One baseline whose (u, v) in metres is (ul, vl) wavelengths at f."""
+    return np.array([[[ul * rime_np.C / f, vl * rime_np.C / f, 0.0]]])
+
+
+def _elliptical_gaussian_image(el, em, R, x):
+    """Gaussian brightness with full width at half maximum |(el, em)| along
+    the direction (el, em) and R |(el, em)| across it."""
+    a = math.hypot(el, em)
+    para = np.array([el, em]) / a
+    perp = np.array([-para[1], para[0]])
+    L, M = np.meshgrid(x, x, indexing='ij')
+    xa = L * para[0] + M * para[1]
+    xp = L * perp[0] + M * perp[1]
+    # exp(-4 ln2 (x / FWHM)^2) is 1/2 at x = FWHM / 2
+    return np.exp(-4.0 * math.log(2.0) * ((xa / a) ** 2 + (xp / (R * a)) ** 2))
+
+
+def test_gaussian_shape_is_fourier_transform_of_elliptical_gaussian():
+    f = 1.4e9
+    rng = np.random.default_rng(11)
+    for el, em, R in ((6e-5, -8e-5, 0.45), (-3e-5, 2e-5, 1.7)):
+        a = math.hypot(el, em)
+        w = a * max(R, 1.0)
+        x = (np.arange(601) - 300) * (5 * w / 300)
+        img = _elliptical_gaussian_image(el, em, R, x)
+        gs = np.array([[el], [em], [R]])
+        for _ in range(5):
+            k = rng.uniform(0.1, 2.5) / w
+            th = rng.uniform(0, 2 * np.pi)
+            ul, vl = k * math.cos(th), k * math.sin(th)
+            ft = _fourier_sum(img, x, ul, vl)
+            got = rime_np.gaussian_shape(_uvw_at(ul, vl, f), gs, np.array([f]))[0, 0, 0, 0]
+            assert abs(ft.imag) < 1e-12
+            assert abs(got - ft.real) < 1e-12, (el, em, R, ul, vl, got, ft.real)
+
+
+def _rot(th):
+    return np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+
+
+def test_gaussian_shape_rotation_covariance_and_axes():
+    """(u, v) and (el, em) rotated together leave the factor unchanged; at
+    R = 1 it depends on |(el, em)| |(u, v)| only; along (el, em) the width is
+    set by |(el, em)| and across it by R |(el, em)|."""
+    rng = np.random.default_rng(12)
+    f = np.array([1.4e9, 1.42e9])
+    uvw = rng.uniform(-3000, 3000, (2, 5, 3))
+    gs = np.array([[7e-5], [-4e-5], [0.6]])
+    g0 = rime_np.gaussian_shape(uvw, gs, f)
+    for th in (0.3, 1.9, -2.4):
+        Rm = _rot(th)
+        uvr = uvw.copy()
+        uvr[..., :2] = uvw[..., :2] @ Rm.T
+        gsr = gs.copy()
+        gsr[:2, 0] = Rm @ gs[:2, 0]
+        np.testing.assert_allclose(rime_np.gaussian_shape(uvr, gsr, f), g0, rtol=1e-12)
+    # R = 1: a round Gaussian, the same for every direction of (el, em) of
+    # the same length
+    round1 = rime_np.gaussian_shape(uvw, np.array([[7e-5], [-4e-5], [1.0]]), f)
+    round2 = rime_np.gaussian_shape(uvw, np.array([[0.0], [math.hypot(7e-5, 4e-5)], [1.0]]), f)
+    np.testing.assert_allclose(round1, round2, rtol=1e-12)
+    # a baseline along (el, em) does not see R; one across it sees R * |a|
+    el, em = 7e-5, -4e-5
+    para = np.array([el, em]) / math.hypot(el, em)
+    perp = np.array([-para[1], para[0]])
+    b = 2000.0
+    for R in (0.2, 3.0):
+        shape = np.array([[el], [em], [R]])
+        along = rime_np.gaussian_shape(np.array([[[*(b * para), 0.0]]]), shape, f)
+        across = rime_np.gaussian_shape(np.array([[[*(b * perp), 0.0]]]), shape, f)
+        ref_round = rime_np.gaussian_shape(np.array([[[*(b * para), 0.0]]]),
+                                           np.array([[el], [em], [1.0]]), f)
+        np.testing.assert_allclose(along, ref_round, rtol=1e-12)
+        # across: the round factor at R times the baseline length
+        across_ref = rime_np.gaussian_shape(np.array([[[*(R * b * para), 0.0]]]),
+                                            np.array([[el], [em], [1.0]]), f)
+        np.testing.assert_allclose(across, across_ref, rtol=1e-12)
+
+
+def test_gaussian_shape_limits():
     uvw_bl = RNG.uniform(-6000, 6000, (2, 3, 3))
     f = np.array([1.4e9, 1.42e9])
     gs = np.array([[1e-4, 0.0], [-5e-5, 0.0], [0.7, 0.0]])   # second source: zero shape
     g = rime_np.gaussian_shape(uvw_bl, gs, f)
     np.testing.assert_allclose(g[1], 1.0)
-    k = rime_np.GAUSS_SCALE
-    el, em, R = gs[:, 0]
-    for t, b, c in np.ndindex(2, 3, 2):
-        u, v = uvw_bl[t, b, :2]
-        # rotate (u, v) into the source frame, squash the major axis by R
-        ua, va = (u * em - v * el), (u * el + v * em)
-        want = math.exp(-((ua * k * f[c] * R) ** 2 + (va * k * f[c]) ** 2))
-        assert abs(g[0, t, b, c] - want) < 1e-14
+    # zero baseline: every shape gives 1
+    np.testing.assert_allclose(rime_np.gaussian_shape(0 * uvw_bl, gs, f), 1.0)
     # symmetric in (u, v) -> (-u, -v): the baseline direction does not matter
     np.testing.assert_allclose(g, rime_np.gaussian_shape(-uvw_bl, gs, f), rtol=1e-14)
     # the constant: fwhm2int*sqrt(2)*pi/C with fwhm2int = 1/sqrt(ln 256)
+    k = rime_np.GAUSS_SCALE
     assert abs(k - math.sqrt(2) * math.pi / 299792458.0 / math.sqrt(8 * math.log(2))) < 1e-25
+
+
+def _shear(e1, e2):
+    return np.array([[1.0 + e1, e2], [e2, 1.0 - e1]])
+
+
+def _sheared_disk_image(e1, e2, R, x):
+    """A round exponential disk exp(-|y| / R') carried onto the sky by the
+    symmetric shear x = S y, S = [[1 + e1, e2], [e2, 1 - e1]], with
+    R' = R / det S."""
+    S = _shear(e1, e2)
+    Rp = R / np.linalg.det(S)
+    Si = np.linalg.inv(S)
+    L, M = np.meshgrid(x, x, indexing='ij')
+    y0 = Si[0, 0] * L + Si[0, 1] * M
+    y1 = Si[1, 0] * L + Si[1, 1] * M
+    return np.exp(-np.hypot(y0, y1) / Rp), Rp
+
+
+def test_sersic_shape_is_fourier_transform_of_sheared_exponential_disk():
+    """The exponential disk's transform has a cusp-limited grid error of
+    ~2e-6 on this grid (40 R' each side, 2500 points); a flipped or dropped
+    shear term moves the factor by > 3e-2 at these frequencies."""
+    f = 1.4e9
+    rng = np.random.default_rng(13)
+    for e1, e2, R in ((0.3, -0.2, 5e-5), (-0.15, 0.35, 3e-5)):
+        Rp = R / np.linalg.det(_shear(e1, e2))
+        x = (np.arange(2500) - 1249.5) * (80 * Rp * 1.4 / 2500)
+        img, _ = _sheared_disk_image(e1, e2, R, x)
+        ss = np.array([[e1], [e2], [R]])
+        for _ in range(3):
+            k = rng.uniform(0.2, 2.0) / (2 * np.pi * Rp)
+            th = rng.uniform(0, 2 * np.pi)
+            ul, vl = k * math.cos(th), k * math.sin(th)
+            ft = _fourier_sum(img, x, ul, vl)
+            got = rime_np.sersic_shape(_uvw_at(ul, vl, f), ss, np.array([f]))[0, 0, 0, 0]
+            assert abs(got - ft.real) < 1e-5, (e1, e2, R, ul, vl, got, ft.real)
+
+
+def test_sersic_shape_spin2_rotation_and_reflection():
+    """Rotating (u, v) by th and (e1, e2) as a spin-2 quantity by 2 th leaves
+    the factor unchanged, and so does reflecting v -> -v with e2 -> -e2."""
+    rng = np.random.default_rng(14)
+    f = np.array([1.4e9, 1.41e9])
+    uvw = rng.uniform(-3000, 3000, (2, 5, 3))
+    ss = np.array([[0.25], [-0.3], [4e-5]])
+    s0 = rime_np.sersic_shape(uvw, ss, f)
+    for th in (0.4, -1.3, 2.8):
+        uvr = uvw.copy()
+        uvr[..., :2] = uvw[..., :2] @ _rot(th).T
+        ssr = ss.copy()
+        ssr[:2, 0] = _rot(2 * th) @ ss[:2, 0]
+        np.testing.assert_allclose(rime_np.sersic_shape(uvr, ssr, f), s0, rtol=1e-12)
+    ref = uvw.copy()
+    ref[..., 1] *= -1
+    ssf = ss.copy()
+    ssf[1] *= -1
+    np.testing.assert_allclose(rime_np.sersic_shape(ref, ssf, f), s0, rtol=1e-12)
 
 
 def test_sersic_shape_limits():
     uvw_bl = RNG.uniform(-6000, 6000, (2, 3, 3))
     f = np.array([1.4e9])
-    ss = np.array([[0.0, 0.0], [0.0, 0.0], [0.0, 1e-4]])
+    ss = np.array([[0.0, 0.0, 0.2], [0.0, 0.0, 0.1], [0.0, 1e-4, 1e-4]])
     g = rime_np.sersic_shape(uvw_bl, ss, f)
     np.testing.assert_allclose(g[0], 1.0)
+    np.testing.assert_allclose(rime_np.sersic_shape(0 * uvw_bl, ss, f), 1.0)
+    # e1 = e2 = 0: the round disk's transform, (1 + (2 pi R k)^2)^(-3/2)
     k = 2 * math.pi / 299792458.0 * f[0] * 1e-4
     for t, b in np.ndindex(2, 3):
         q2 = (uvw_bl[t, b, 0] ** 2 + uvw_bl[t, b, 1] ** 2) * k * k
         assert abs(g[1, t, b, 0] - (1 + q2) ** -1.5) < 1e-14
+    np.testing.assert_allclose(g, rime_np.sersic_shape(-uvw_bl, ss, f), rtol=1e-14)
 
 
 # ---- per-baseline products, gains, flags, chi^2 -------------------------
@@ -376,11 +528,48 @@ def test_planted_mistakes_are_caught(monkeypatch):
     assert _fails(test_point_source_fourier_relation)
     monkeypatch.setattr(rime_np, 'coherency_sum', good_cs)
 
-    # and the unplanted oracle passes all four
+    # 5. el and em swapped in the gaussian shape
+    good_g = rime_np.gaussian_shape
+
+    def g_swapped(uvw_bl, gauss_shape, frequency, *args, **kw):
+        return good_g(uvw_bl, np.asarray(gauss_shape)[[1, 0, 2]], frequency, *args, **kw)
+    monkeypatch.setattr(rime_np, 'gaussian_shape', g_swapped)
+    assert _fails(test_gaussian_shape_is_fourier_transform_of_elliptical_gaussian)
+    assert _fails(test_gaussian_shape_rotation_covariance_and_axes)
+    monkeypatch.setattr(rime_np, 'gaussian_shape', good_g)
+
+    # 6. the sign of e2 flipped in u1: u1 = u (1 + e1) - v e2
+    good_s = rime_np.sersic_shape
+
+    def s_flipped(uvw_bl, sersic_shape, frequency, two_pi_over_c=rime_np.TWO_PI_OVER_C):
+        e1, e2, R = (np.asarray(x, np.float64)[:, None, None, None] for x in sersic_shape)
+        u, v = uvw_bl[None, ..., 0, None], uvw_bl[None, ..., 1, None]
+        s = two_pi_over_c * np.asarray(frequency) * R / (1 - e1 * e1 - e2 * e2)
+        den = 1 + ((u * (1 + e1) - v * e2) * s) ** 2 + ((u * e2 + v * (1 - e1)) * s) ** 2
+        return den ** -1.5
+    monkeypatch.setattr(rime_np, 'sersic_shape', s_flipped)
+    assert _fails(test_sersic_shape_is_fourier_transform_of_sheared_exponential_disk)
+    assert _fails(test_sersic_shape_spin2_rotation_and_reflection)
+    monkeypatch.setattr(rime_np, 'sersic_shape', good_s)
+
+    # 7. a dropped term: R' = R without the 1 / (1 - e1^2 - e2^2) scaling
+    def s_no_det(uvw_bl, sersic_shape, frequency, *args, **kw):
+        e1, e2, R = np.asarray(sersic_shape, np.float64)
+        return good_s(uvw_bl, np.stack([e1, e2, R * (1 - e1 * e1 - e2 * e2)]), frequency,
+                      *args, **kw)
+    monkeypatch.setattr(rime_np, 'sersic_shape', s_no_det)
+    assert _fails(test_sersic_shape_is_fourier_transform_of_sheared_exponential_disk)
+    monkeypatch.setattr(rime_np, 'sersic_shape', good_s)
+
+    # and the unplanted oracle passes all of them
     test_k_jones_brute_force_and_unit_modulus()
     test_b_sqrt_matches_scipy_sqrtm_and_invariants()
     test_rime_brute_force(False, False)
     test_point_source_fourier_relation()
+    test_gaussian_shape_is_fourier_transform_of_elliptical_gaussian()
+    test_gaussian_shape_rotation_covariance_and_axes()
+    test_sersic_shape_is_fourier_transform_of_sheared_exponential_disk()
+    test_sersic_shape_spin2_rotation_and_reflection()
 
 
 def test_parallel_oracle_matches_single_process():
