@@ -45,6 +45,13 @@ class RimeChi2(torch.autograd.Function):
 
 def chi2(arrays, counts, use_weight_vector=False, sigma_sqrd=None):
     """X2 (0-d float64 tensor) of the arrays dict; differentiable in the
-    parameter tensors of the dict that require grad."""
+    parameter tensors of the dict that require grad.
+
+    Raises ValueError before the forward when gradients are wanted for an
+    antenna layout the adjoint does not take (pairs changing with time, or
+    one pair on two baselines)."""
+    if torch.is_grad_enabled() and any(
+            isinstance(arrays.get(n), torch.Tensor) and arrays[n].requires_grad for n in PARAMS):
+        rime.baseline_map(arrays)
     return RimeChi2.apply(*(arrays[n] for n in PARAMS), arrays, tuple(counts),
                           use_weight_vector, sigma_sqrd)

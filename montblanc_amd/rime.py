@@ -47,31 +47,131 @@ def complex_dtype(arrays):
 
 
 def to_device(a, device='cuda:0', dtype=None):
-    """numpy dict (v4 names) -> dict of contiguous device tensors.
+    """dict of numpy arrays or torch tensors (v4 names) -> dict of
+    contiguous device tensors.
 
-    dtype: np.float32 / np.float64 for the float arrays (complex follow);
-    None keeps each array's own dtype. Scalars (beam extents, constants)
-    are kept as Python floats.
+    dtype: np.float32 / np.float64 (or the torch equivalents) for the float
+    arrays, complex arrays following; None keeps each array's own dtype.
+    Antenna indices always become int32 and flags uint8, whatever the input
+    type (torch's default integer type is int64, which the kernels would
+    read as pairs of int32). Scalars (beam extents, constants) are kept as
+    Python floats.
     """
     out = {}
+    if isinstance(dtype, torch.dtype):
+        dtype = {torch.float32: np.float32, torch.float64: np.float64}[dtype]
     ft = None if dtype is None else np.dtype(dtype)
     ct = None if ft is None else (np.complex64 if ft == np.float32 else np.complex128)
+    tft = None if ft is None else (torch.float32 if ft == np.float32 else torch.float64)
+    tct = None if ft is None else (torch.complex64 if ft == np.float32 else torch.complex128)
     for k, v in a.items():
         if isinstance(v, np.ndarray):
-            if v.dtype.kind == 'f' and ft is not None:
-                v = v.astype(ft)
-            elif v.dtype.kind == 'c' and ct is not None:
-                v = v.astype(ct)
-            elif k in INT_ARRAYS:
+            if k in INT_ARRAYS:
                 v = v.astype(np.int32)
             elif k == 'flag':
                 v = v.astype(np.uint8)
+            elif v.dtype.kind == 'f' and ft is not None:
+                v = v.astype(ft)
+            elif v.dtype.kind == 'c' and ct is not None:
+                v = v.astype(ct)
             out[k] = torch.from_numpy(np.ascontiguousarray(v)).to(device)
         elif isinstance(v, torch.Tensor):
+            if k in INT_ARRAYS:
+                v = v.to(torch.int32)
+            elif k == 'flag':
+                v = v.to(torch.uint8)
+            elif v.is_floating_point() and tft is not None:
+                v = v.to(tft)
+            elif v.is_complex() and tct is not None:
+                v = v.to(tct)
             out[k] = v.to(device).contiguous()
         else:
             out[k] = v
     return out
+
+
+# v4 array shapes in terms of the dimension names (v4/config.py:131-247)
+ARRAY_SHAPES = {
+    'uvw': ('ntime', 'na', 3), 'lm': ('nsrc', 2), 'stokes': ('nsrc', 'ntime', 4),
+    'alpha': ('nsrc', 'ntime'), 'frequency': ('nchan',), 'ref_frequency': ('nchan',),
+    'parallactic_angles': ('ntime', 'na'), 'point_errors': ('ntime', 'na', 'nchan', 2),
+    'antenna_scaling': ('na', 'nchan', 2), 'E_beam': ('beam_lw', 'beam_mh', 'beam_nud', 4),
+    'gauss_shape': (3, 'ngsrc'), 'sersic_shape': (3, 'nssrc'),
+    'antenna1': ('ntime', 'nbl'), 'antenna2': ('ntime', 'nbl'),
+    'flag': ('ntime', 'nbl', 'nchan', 4), 'weight_vector': ('ntime', 'nbl', 'nchan', 4),
+    'observed_vis': ('ntime', 'nbl', 'nchan', 4), 'G_term': ('ntime', 'na', 'nchan', 4),
+    'model_vis': ('ntime', 'nbl', 'nchan', 4), 'ekb_vis': ('ntime', 'nbl', 'nchan', 4),
+    'chi_sqrd_result': ('ntime', 'nbl', 'nchan'),
+}
+
+
+def _sizes(arrays, counts):
+    ntime, na = arrays['uvw'].shape[:2]
+    lw, mh, nud = arrays['E_beam'].shape[:3] if arrays.get('E_beam') is not None else (0, 0, 0)
+    return dict(ntime=ntime, na=na, nbl=arrays['antenna1'].shape[1],
+                nchan=arrays['frequency'].shape[0], nsrc=sum(counts), npsrc=counts[0],
+                ngsrc=counts[1], nssrc=counts[2], beam_lw=lw, beam_mh=mh, beam_nud=nud)
+
+
+def check_arrays(arrays, counts, names=None, outputs=None, dims=None):
+    """Raise ValueError unless every array the kernels will read or write
+    has the run dtype, lives on the device of ``uvw``, is contiguous and
+    has the v4 shape implied by uvw (ntime, na), antenna1 (nbl), frequency
+    (nchan), E_beam (beam) and counts = (npsrc, ngsrc, nssrc). Runs before
+    any launch, so a bad argument never reaches a kernel as a raw pointer.
+
+    names: the input names to check (default: every input present);
+    outputs: a dict of output tensors to check the same way;
+    dims: an mbx_dims whose local sizes must agree with the arrays."""
+    for k in ('uvw', 'antenna1', 'frequency'):
+        if not isinstance(arrays.get(k), torch.Tensor):
+            raise ValueError('array %r is missing or not a torch tensor' % k)
+    ft = arrays['uvw'].dtype
+    if ft not in (torch.float32, torch.float64):
+        raise ValueError('uvw must be float32 or float64, not %s' % ft)
+    ct = torch.complex64 if ft == torch.float32 else torch.complex128
+    dev = arrays['uvw'].device
+    if len(counts) != 3 or min(counts) < 0:
+        raise ValueError('counts must be (npsrc, ngsrc, nssrc) >= 0, got %r' % (counts,))
+    sz = _sizes(arrays, counts)
+    items = [(n, arrays.get(n)) for n in (names if names is not None else _lib.INPUT_NAMES)]
+    items += list((outputs or {}).items())
+    for n, t in items:
+        if t is None or n not in ARRAY_SHAPES:
+            continue
+        if not isinstance(t, torch.Tensor):
+            raise ValueError('array %r must be a torch tensor, not %s' % (n, type(t).__name__))
+        if n in INT_ARRAYS:
+            want = torch.int32
+        elif n == 'flag':
+            want = torch.uint8
+        elif n in COMPLEX_ARRAYS or n in ('model_vis', 'ekb_vis'):
+            want = ct
+        else:
+            want = ft
+        if t.dtype != want:
+            raise ValueError('array %r has dtype %s; this call needs %s (the dtype of uvw '
+                             'decides float32 or float64)' % (n, t.dtype, want))
+        if t.device != dev:
+            raise ValueError('array %r is on %s, uvw on %s' % (n, t.device, dev))
+        if not t.is_contiguous():
+            raise ValueError('array %r is not contiguous' % n)
+        shape = tuple(s if isinstance(s, int) else sz[s] for s in ARRAY_SHAPES[n])
+        if n in ('gauss_shape', 'sersic_shape') and shape[1] == 0 and t.numel() == 0:
+            continue
+        if tuple(t.shape) != shape:
+            raise ValueError('array %r has shape %s; the dimensions give %s' %
+                             (n, tuple(t.shape), shape))
+    x2 = (outputs or {}).get('X2')
+    if x2 is not None and (x2.dtype != torch.float64 or x2.device != dev):
+        raise ValueError('X2 must be a float64 scalar tensor on %s' % dev)
+    if dims is not None:
+        # extents against global and local sizes are validated by the library
+        for n in ('ntime', 'na', 'nbl', 'nchan', 'npsrc', 'ngsrc', 'nssrc'):
+            dm = getattr(dims, n)
+            if dm.local_size != sz[n]:
+                raise ValueError('dims.%s.local_size = %d but the arrays give %d' %
+                                 (n, dm.local_size, sz[n]))
 
 
 def make_dims(arrays, counts, extents=None, global_sizes=None):
@@ -220,7 +320,10 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
     workspace (a Workspace) replaces the per-device one, for concurrent
     calls on several streams."""
     dev = arrays['uvw'].device
+    check_arrays(arrays, counts, dims=dims)
     out = outputs if outputs is not None else alloc_outputs(arrays, keep_ekb)
+    if outputs is not None:
+        check_arrays(arrays, counts, names=(), outputs=out)
     d = make_dims(arrays, counts) if dims is None else dims
     plan = pair_plan(arrays, pair_block, point_gemm)
     ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
@@ -241,11 +344,24 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
 
 def baseline_map(arrays):
     """Device (na, na) pair -> baseline map of the antenna layout (cached in
-    the dict under '_baseline_map')."""
+    the dict under '_baseline_map').
+
+    Raises ValueError for layouts the adjoint does not take: antenna pairs
+    that change with time, or a pair that appears on two baselines. The
+    forward accepts those (on its per-baseline kernel); the adjoint's
+    source passes work on the per-(time, channel) antenna-pair matrix,
+    which needs one baseline per pair for the whole run."""
     m = arrays.get('_baseline_map')
     if m is None:
-        host = _lib.build_baseline_map(arrays['antenna1'].cpu().numpy(),
-                                       arrays['antenna2'].cpu().numpy(), arrays['uvw'].shape[1])
+        try:
+            host = _lib.build_baseline_map(arrays['antenna1'].cpu().numpy(),
+                                           arrays['antenna2'].cpu().numpy(),
+                                           arrays['uvw'].shape[1])
+        except _lib.MontblancHipError as e:
+            if e.code != _lib.E_UNSUPPORTED:
+                raise
+            raise ValueError('gradients need an antenna layout that is the same at every time '
+                             'step with at most one baseline per antenna pair (%s)' % e) from None
         m = torch.from_numpy(host).to(arrays['uvw'].device)
         arrays['_baseline_map'] = m
     return m
@@ -265,15 +381,34 @@ def alloc_grads(arrays, counts):
                 G_term=torch.empty((ntime, na, nchan, 4), dtype=torch.complex128, device=dev))
 
 
+def _check_grads(arrays, counts, grads):
+    sz = _sizes(arrays, counts)
+    want = dict(lm=(sz['nsrc'], 2), stokes=(sz['nsrc'], sz['ntime'], 4),
+                alpha=(sz['nsrc'], sz['ntime']), gauss_shape=(3, sz['ngsrc']),
+                sersic_shape=(3, sz['nssrc']), G_term=(sz['ntime'], sz['na'], sz['nchan'], 4))
+    dev = arrays['uvw'].device
+    for n, shape in want.items():
+        t = grads.get(n)
+        if t is None:
+            continue
+        dt = torch.complex128 if n == 'G_term' else torch.float64
+        if (not isinstance(t, torch.Tensor) or t.dtype != dt or t.device != dev
+                or not t.is_contiguous() or tuple(t.shape) != shape):
+            raise ValueError('gradient buffer %r must be a contiguous %s tensor of shape %s on %s'
+                             % (n, dt, shape, dev))
+
+
 def adjoint(arrays, counts, ekb_vis, grad_scale=1.0, use_weight_vector=False, sigma_sqrd=None,
             grads=None, stream=None, dims=None, workspace=None):
     """d chi^2 / d(lm, stokes, alpha, gauss_shape, sersic_shape, G_term) times
     grad_scale, from the forward's pre-gain sums ``ekb_vis``. Returns a dict
     of float64 tensors (G_term complex128: d/dRe + i d/dIm)."""
     dev = arrays['uvw'].device
-    grads = alloc_grads(arrays, counts) if grads is None else grads
-    d = make_dims(arrays, counts) if dims is None else dims
+    check_arrays(arrays, counts, outputs={'ekb_vis': ekb_vis}, dims=dims)
     bmap = baseline_map(arrays)
+    grads = alloc_grads(arrays, counts) if grads is None else grads
+    _check_grads(arrays, counts, grads)
+    d = make_dims(arrays, counts) if dims is None else dims
     ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
     g = _lib.mbx_grads(**{n: _raw(grads.get(n)) for n in _lib.GRAD_NAMES})
     lib = _lib.load()
@@ -300,11 +435,23 @@ def forward_numpy(a, counts, device='cuda:0', **kw):
 
 # ---- stage functions (one v4 node each) ---------------------------------
 
+def _check_out(arrays, t, shape):
+    """A per-source scratch array (B_sqrt, jones) of the run's complex type."""
+    ct = complex_dtype(arrays)
+    dev = arrays['uvw'].device
+    if (not isinstance(t, torch.Tensor) or t.dtype != ct or t.device != dev
+            or not t.is_contiguous() or tuple(t.shape) != tuple(shape)):
+        raise ValueError('scratch array must be a contiguous %s tensor of shape %s on %s'
+                         % (ct, tuple(shape), dev))
+
+
 def b_sqrt(arrays, counts, out=None):
     ns, nt = arrays['stokes'].shape[:2]
     nc = arrays['frequency'].shape[0]
+    check_arrays(arrays, counts, names=('stokes', 'alpha', 'frequency', 'ref_frequency'))
     out = out if out is not None else torch.empty((ns, nt, nc, 4), dtype=complex_dtype(arrays),
                                                   device=arrays['uvw'].device)
+    _check_out(arrays, out, (ns, nt, nc, 4))
     d = make_dims(arrays, counts)
     _lib.call('mbx_b_sqrt_' + suffix(arrays), ctypes.byref(d), _ptr(arrays['stokes']),
               _ptr(arrays['alpha']), _ptr(arrays['frequency']), _ptr(arrays['ref_frequency']),
@@ -316,8 +463,11 @@ def e_beam(arrays, counts, out=None):
     ns = arrays['lm'].shape[0]
     nt, na = arrays['uvw'].shape[:2]
     nc = arrays['frequency'].shape[0]
+    check_arrays(arrays, counts, names=('lm', 'parallactic_angles', 'point_errors',
+                                        'antenna_scaling', 'frequency', 'E_beam'))
     out = out if out is not None else torch.empty((ns, nt, na, nc, 4), dtype=complex_dtype(arrays),
                                                   device=arrays['uvw'].device)
+    _check_out(arrays, out, (ns, nt, na, nc, 4))
     d = make_dims(arrays, counts)
     b = beam_extents(arrays)
     _lib.call('mbx_e_beam_' + suffix(arrays), ctypes.byref(d), _ptr(arrays['lm']),
@@ -328,6 +478,10 @@ def e_beam(arrays, counts, out=None):
 
 
 def ekb_sqrt(arrays, counts, B_sqrt, jones):
+    check_arrays(arrays, counts, names=('uvw', 'lm', 'frequency'))
+    ns, (nt, na), nc = sum(counts), arrays['uvw'].shape[:2], arrays['frequency'].shape[0]
+    _check_out(arrays, B_sqrt, (ns, nt, nc, 4))
+    _check_out(arrays, jones, (ns, nt, na, nc, 4))
     d = make_dims(arrays, counts)
     k = consts(arrays)
     _lib.call('mbx_ekb_sqrt_' + suffix(arrays), ctypes.byref(d), _ptr(arrays['uvw']),
@@ -338,6 +492,12 @@ def ekb_sqrt(arrays, counts, B_sqrt, jones):
 
 def sum_coherencies(arrays, counts, jones, model_vis, chi_sqrd_result, residual=False,
                     use_weight_vector=False, dims=None):
+    check_arrays(arrays, counts, names=('uvw', 'gauss_shape', 'sersic_shape', 'frequency',
+                                        'antenna1', 'antenna2', 'flag', 'weight_vector',
+                                        'observed_vis', 'G_term'),
+                 outputs=dict(model_vis=model_vis, chi_sqrd_result=chi_sqrd_result), dims=dims)
+    ns, (nt, na), nc = sum(counts), arrays['uvw'].shape[:2], arrays['frequency'].shape[0]
+    _check_out(arrays, jones, (ns, nt, na, nc, 4))
     d = make_dims(arrays, counts) if dims is None else dims
     k = consts(arrays)
     _lib.call('mbx_sum_coherencies_' + suffix(arrays), ctypes.byref(d), _ptr(arrays['uvw']),
@@ -351,6 +511,8 @@ def sum_coherencies(arrays, counts, jones, model_vis, chi_sqrd_result, residual=
 
 def chi2_reduce(arrays, counts, chi_sqrd_result, X2, use_weight_vector=False, sigma_sqrd=None,
                 dims=None):
+    check_arrays(arrays, counts, names=(), outputs=dict(chi_sqrd_result=chi_sqrd_result, X2=X2),
+                 dims=dims)
     d = make_dims(arrays, counts) if dims is None else dims
     k = consts(arrays, sigma_sqrd)
     lib = _lib.load()
