@@ -70,10 +70,36 @@ def rime_solver(slvr_cfg):
     else:
         raise ValueError('Invalid version %s' % version)
     data_source = cfg.get(RimeSolverConfig.DATA_SOURCE, RimeSolverConfig.DATA_SOURCE_MS)
-    if data_source not in (RimeSolverConfig.DATA_SOURCE_MS, RimeSolverConfig.DATA_SOURCE_TEST,
-                           RimeSolverConfig.DATA_SOURCE_DEFAULT, RimeSolverConfig.DATA_SOURCE_EMPTY):
+    if data_source == RimeSolverConfig.DATA_SOURCE_MS:
+        return create_rime_solver_from_ms(cls, cfg)
+    if data_source not in (RimeSolverConfig.DATA_SOURCE_TEST, RimeSolverConfig.DATA_SOURCE_DEFAULT,
+                           RimeSolverConfig.DATA_SOURCE_EMPTY):
         raise ValueError('Invalid data source %s' % data_source)
     return cls(cfg)
+
+
+def create_rime_solver_from_ms(slvr_class_type, slvr_cfg):
+    """A solver of ``slvr_class_type`` filled from the MeasurementSet named by
+    slvr_cfg['msfile'] (montblanc/factory.py:113-148): KeyError without one;
+    the dimensions (ntime, na, nbl, nbands, nchan) come from the loader
+    into the configuration, the solver is built on them and the loader
+    transfers the arrays into it. The tables are opened by
+    ``ms_loader.open_tables`` (python-casacore)."""
+    from . import ms_loader
+    cfg = dict(slvr_cfg)
+    O = RimeSolverConfig
+    if cfg.get(O.MS_FILE) is None:
+        raise KeyError('%s key is set to %s in the Solver Configuration, but no MeasurementSet '
+                       'file has been specified in the %s key'
+                       % (O.DATA_SOURCE, O.DATA_SOURCE_MS, O.MS_FILE))
+    with ms_loader.MeasurementSetLoader(cfg[O.MS_FILE],
+                                        auto_correlations=cfg.get(O.AUTO_CORRELATIONS, False)) as ld:
+        ntime, nbl, na, nbands, nchan = ld.get_dims()
+        cfg[O.NTIME], cfg[O.NA], cfg[O.NBL] = ntime, na, nbl
+        cfg[O.NBANDS], cfg[O.NCHAN] = nbands, nchan
+        slvr = slvr_class_type(cfg)
+        ld.load(slvr, cfg)
+        return slvr
 
 
 def __getattr__(name):

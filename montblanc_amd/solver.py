@@ -328,13 +328,10 @@ class RimeSolver(object):
     def create_arrays(self, ignore=None, supplied=None):
         ignore = set(ignore or [])
         supplied = supplied or {}
+        # 'ms' (the factory transfers the MeasurementSet's arrays after
+        # construction) and 'default' start from the schema defaults
+        # (mb_cuda_solver.py:43-61)
         source = self._slvr_cfg.get(Options.DATA_SOURCE, Options.DATA_SOURCE_DEFAULT)
-        if source == Options.DATA_SOURCE_MS:
-            raise NotImplementedError('data_source ms: build the solver with data_source '
-                                      'default/test/empty and load the MeasurementSet with '
-                                      'montblanc_amd.ms_loader.MeasurementSetLoader(msfile).load('
-                                      'solver, cfg) (needs python-casacore), or transfer_<array>() '
-                                      'the columns turned into arrays by ms_loader.ms_arrays()')
         for name, a in self._arrays.items():
             if name in ignore:
                 continue
