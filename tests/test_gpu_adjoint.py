@@ -4,19 +4,22 @@ The reference gradient is torch float64 autograd of oracle/rime_torch.py,
 itself pinned to central finite differences of the numpy oracle
 (tests/test_oracle_grad.py). float64 gradients must agree to 1e-10
 relative per parameter array (max-norm); float32 ones (float64 phase and
-beam coordinates) to 1e-4 relative, since chi^2's derivatives carry the
-residual O - M, which cancels in float32 when the model fits.
+beam coordinates) within the derived first-order bound of
+tests/grad_bound.py: the float32 model's measured error and the Jones
+recomputation's error carried through chi^2's derivative, by
+Cauchy-Schwarz over the visibilities, element by element.
 """
 
 import numpy as np
 import pytest
 import torch
 
+from grad_bound import check_f32_gradients
 from oracle import rime_np, rime_torch, synth
 
 pytestmark = pytest.mark.gpu
 
-TOL = {np.float64: 1e-10, np.float32: 1e-4}
+TOL = {np.float64: 1e-10}
 NAMES = ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape', 'G_term')
 
 
@@ -45,7 +48,24 @@ def gpu_grads(a, counts, dtype, use_w):
     out = rime.forward(arrays, counts, use_weight_vector=use_w, keep_ekb=True)
     g = rime.adjoint(arrays, counts, out['ekb_vis'], use_weight_vector=use_w)
     torch.cuda.synchronize()
-    return {k: v.cpu().numpy() for k, v in g.items()}, float(out['X2'].item())
+    got = {k: v.cpu().numpy() for k, v in g.items()}
+    got['model_vis'] = out['model_vis'].cpu().numpy()
+    return got, float(out['X2'].item())
+
+
+def check_grads(a, counts, dtype, got, ref, use_w=False, sigma_sqrd=1.3, names=NAMES):
+    """float64: 1e-10 relative per array; float32: the derived bound."""
+    if dtype == np.float32:
+        up = synth.upcast(synth.as_dtype(a, dtype))
+        print(check_f32_gradients(up, counts, got, ref, got['model_vis'], use_weight_vector=use_w,
+                                  sigma_sqrd=sigma_sqrd, names=names))
+        return
+    for name in names:
+        if ref[name] is None or ref[name].size == 0:
+            continue
+        e = rel(got[name], ref[name])
+        print(name, e)
+        assert e < TOL[dtype], (name, e)
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
@@ -56,13 +76,18 @@ def test_adjoint_matches_reference_gradient(dtype, counts, weights):
     ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts,
                                use_weight_vector=weights, sigma_sqrd=1.3)
     got, x2 = gpu_grads(a, counts, dtype, weights)
-    assert abs(x2 - ref['X2']) < max(TOL[dtype], 1e-10) * ref['X2'] * (1 if dtype == np.float64 else 10)
-    for name in NAMES:
-        if ref[name] is None or ref[name].size == 0:
-            continue
-        e = rel(got[name], ref[name])
-        print(name, e)
-        assert e < TOL[dtype], (name, e)
+    if dtype == np.float64:
+        assert abs(x2 - ref['X2']) < TOL[dtype] * ref['X2']
+    else:
+        # |X2~ - X2| <= (2 w_max / s)(||R|| ||dM|| + ||dM||^2 / 2) + 8 u X2
+        up = synth.upcast(synth.as_dtype(a, dtype))
+        keep = up['flag'] == 0
+        M = rime_np.rime(up, *counts)['model_vis'] * keep
+        dM = np.linalg.norm((got['model_vis'] - M) * keep)
+        R = np.linalg.norm((up['observed_vis'] - M) * keep)
+        wmax, s = (up['weight_vector'].max(), 1.0) if weights else (1.0, 1.3)
+        assert abs(x2 - ref['X2']) <= 2 * wmax / s * (R * dM + dM * dM / 2) + 8 * 2.0 ** -24 * ref['X2']
+    check_grads(a, counts, dtype, got, ref, use_w=weights)
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
@@ -71,8 +96,7 @@ def test_adjoint_autocorrelations_and_single_channel(dtype):
     a = problem(5, 2, 1, counts, seed=41, auto=True)
     ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts, sigma_sqrd=1.3)
     got, _ = gpu_grads(a, counts, dtype, False)
-    for name in ('lm', 'stokes', 'alpha', 'gauss_shape', 'G_term'):
-        assert rel(got[name], ref[name]) < TOL[dtype], name
+    check_grads(a, counts, dtype, got, ref, names=('lm', 'stokes', 'alpha', 'gauss_shape', 'G_term'))
 
 
 @pytest.mark.parametrize('dtype,na,counts', [
@@ -89,15 +113,7 @@ def test_adjoint_many_point_tiles(dtype, na, counts):
     a = problem(na, 2, 3, counts, seed=na + counts[0])
     ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts, sigma_sqrd=1.3)
     got, _ = gpu_grads(a, counts, dtype, False)
-    # float32: each gradient sums na * ntime * nchan item terms of both signs
-    # per source, so its rounding grows with the problem (about 1e-4 here)
-    tol = TOL[dtype] if dtype == np.float64 else 3e-4
-    for name in NAMES:
-        if ref[name] is None or ref[name].size == 0:
-            continue
-        e = rel(got[name], ref[name])
-        print(name, e)
-        assert e < tol, (name, e)
+    check_grads(a, counts, dtype, got, ref)
 
 
 @pytest.mark.parametrize('dtype,na', [(np.float64, 27), (np.float32, 27), (np.float64, 40),
@@ -112,13 +128,7 @@ def test_adjoint_many_shaped_tiles(dtype, na):
     a = problem(na, 1, 2, counts, seed=77 + na)
     ref = rime_torch.gradients(synth.upcast(synth.as_dtype(a, dtype)), *counts, sigma_sqrd=1.3)
     got, _ = gpu_grads(a, counts, dtype, False)
-    tol = TOL[dtype] if dtype == np.float64 else 3e-4
-    for name in NAMES:
-        if ref[name] is None or ref[name].size == 0:
-            continue
-        e = rel(got[name], ref[name])
-        print(name, e)
-        assert e < tol, (name, e)
+    check_grads(a, counts, dtype, got, ref)
 
 
 def test_autograd_function_and_gpu_finite_differences():
@@ -172,5 +182,7 @@ def test_adjoint_rejects_partial_source_batch_and_duplicates():
     a2 = arrays['antenna2'].clone()
     a1[:, 1], a2[:, 1] = a1[:, 0], a2[:, 0]
     dup['antenna1'], dup['antenna2'] = a1, a2
-    with pytest.raises(_lib.MontblancHipError):
+    # raised on the host before any launch (the layout builder's
+    # MBX_E_UNSUPPORTED, as a ValueError naming the layout rule)
+    with pytest.raises(ValueError, match='one baseline per antenna pair'):
         rime.adjoint(dup, counts, out['ekb_vis'])

@@ -151,13 +151,13 @@ def test_forward_rejects_bad_arguments():
     from montblanc_amd import _lib, rime
     a = synth.make_inputs(4, 1, 2, 1, 0, 0, beam=(3, 3, 3), seed=1)
     arrays = rime.to_device(a)
-    with pytest.raises(_lib.MontblancHipError) as e:
-        rime.forward(arrays, (1, 1, 0))       # gaussian count without gauss_shape data
-    assert e.value.code < 0
+    with pytest.raises(ValueError, match='lm'):
+        rime.forward(arrays, (1, 1, 0))       # counts disagree with the source arrays
     bad = dict(arrays)
     bad['beam_ul'] = bad['beam_ll']
-    with pytest.raises(_lib.MontblancHipError):
+    with pytest.raises(_lib.MontblancHipError) as e:   # the library's own validation
         rime.forward(bad, (1, 0, 0))
+    assert e.value.code < 0
 
 
 VARIANTS = ['generic', (1, 1, -1), (2, 2, -1), (2, 4, -1), (1, 1, 0), (2, 2, 0), (2, 2, 3),
