@@ -229,6 +229,25 @@ int mbx_forward_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *
                     const mbx_pair_plan *plan, int options,
                     void *workspace, size_t workspace_bytes, void *stream);
 
+/* The fused forward without a caller workspace. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results as mbx_forward_f32/f64 minus the
+ * workspace, which here is a library-held device array of
+ * MBX_STATIC_SCRATCH_BYTES per device (each device that runs the library
+ * keeps it until the process ends); dimensions whose
+ * mbx_forward_workspace_bytes exceed it are rejected with
+ * MBX_E_WORKSPACE before anything runs. The stream's device must be the
+ * current one. Allocates nothing. Calls that overlap on one device share
+ * the array and race: serialise them (one stream, or events). */
+#define MBX_STATIC_SCRATCH_BYTES (64u << 20)
+int mbx_forward_static_f32(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                           const mbx_beam_extents *beam, const mbx_consts *k,
+                           const mbx_pair_plan *plan, int options, void *stream);
+int mbx_forward_static_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                           const mbx_beam_extents *beam, const mbx_consts *k,
+                           const mbx_pair_plan *plan, int options, void *stream);
+
 /* ---- Adjoint: d chi^2 / d(parameters), recomputing the Jones terms ----
  * No reference counterpart (the reference has no gradient). */
 

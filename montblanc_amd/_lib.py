@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, 'libmontblanc_amd.so')
 ABI_VERSION = 3
 APPLY_WEIGHTS = 1
 RESIDUAL_OUTPUT = 2
+# size of the library-held workspace of mbx_forward_static_* per device
+STATIC_SCRATCH_BYTES = 64 << 20
 
 ERROR_NAMES = {
     -1: 'MBX_E_NULL', -2: 'MBX_E_DIMS', -3: 'MBX_E_EXTENT',
@@ -132,6 +134,7 @@ for _t in ('f32', 'f64'):
         'mbx_sum_coherencies_' + _t: ([_PD] + [_P] * 13 + [_PK, _I, _P], _I),
         'mbx_chi2_reduce_' + _t: ([_PD, _P, _P, _PK, _I, _P, _SZ, _P], _I),
         'mbx_forward_' + _t: ([_PD, _PI, _PO, _PB, _PK, _PP, _I, _P, _SZ, _P], _I),
+        'mbx_forward_static_' + _t: ([_PD, _PI, _PO, _PB, _PK, _PP, _I, _P], _I),
         'mbx_adjoint_' + _t: ([_PD, _PI, _P, _P, ctypes.POINTER(mbx_grads), _PB, _PK, _I,
                                ctypes.c_double, _P, _SZ, _P], _I),
         'mbx_batched_' + _t: ([_PD, _PI, ctypes.POINTER(mbx_batch), _P, ctypes.POINTER(mbx_grads),

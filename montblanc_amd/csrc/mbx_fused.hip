@@ -23,6 +23,7 @@
 // and the epilogue is skipped.
 
 #include <algorithm>
+#include <mutex>
 
 #include "mbx_gemm.hpp"
 
@@ -667,11 +668,85 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
     return r;
 }
 
+// ---- forward with a library-held workspace (no allocation) -------------
+//
+// The workspace of mbx_forward_static_* (beam planes, then the chi^2
+// partials) lives in this file-scope device array: each device that runs
+// the library holds its own copy until the process ends. Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in
+// for the symbol): the device address comes from hipGetSymbolAddress, once
+// per device with that device current, and is kept per device. Calls that
+// overlap on one device share the array and race; every byte the forward
+// reads from it is written earlier in the same call (beam planes by
+// k_beam_planes, each partial by its own workgroup), so nothing needs
+// clearing between calls.
+
+static __device__ __align__(256) unsigned char scratch[MBX_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+static int scratch_on(hipStream_t stream, void **out) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return hip_status(e, "forward_static: hipGetDevice");
+    hipDevice_t sdev = cur;
+    if (stream) {
+        e = hipStreamGetDevice(stream, &sdev);
+        if (e != hipSuccess) return hip_status(e, "forward_static: hipStreamGetDevice");
+    }
+    if (sdev != cur)
+        return set_error(MBX_E_UNSUPPORTED, "forward_static: the stream is on device %d but device "
+                         "%d is current; make the stream's device current", int(sdev), cur);
+    if (cur < 0 || cur >= kMaxDevices)
+        return set_error(MBX_E_UNSUPPORTED, "forward_static: device %d beyond %d", cur, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[cur]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_status(e, "forward_static: hipGetSymbolAddress(scratch)");
+        if (!p) return set_error(MBX_E_NO_DEVICE, "forward_static: no device address for scratch");
+        g_scratch_ptr[cur] = p;
+    }
+    *out = g_scratch_ptr[cur];
+    return MBX_OK;
+}
+
+template <typename T>
+static int forward_static(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                          const mbx_beam_extents *b, const mbx_consts *k, const mbx_pair_plan *pp,
+                          int options, hipStream_t stream) {
+    if (!d) return set_error(MBX_E_NULL, "forward_static: NULL dims");
+    int r = check_dims(d, true);
+    if (r) return r;
+    const size_t need = forward_ws_bytes<T>(*d);
+    if (need > size_t(MBX_STATIC_SCRATCH_BYTES))
+        return set_error(MBX_E_WORKSPACE, "forward_static: the workspace of these dimensions is %zu "
+                         "bytes, beyond the static scratch of %zu bytes; use mbx_forward_%s with a "
+                         "caller workspace", need, size_t(MBX_STATIC_SCRATCH_BYTES),
+                         sizeof(T) == 8 ? "f64" : "f32");
+    void *ws = nullptr;
+    if ((r = scratch_on(stream, &ws))) return r;
+    return forward<T>(d, in, out, b, k, pp, options, ws, size_t(MBX_STATIC_SCRATCH_BYTES), stream);
+}
+
 }  // namespace mbx
 
 using namespace mbx;
 
 extern "C" {
+
+int mbx_forward_static_f32(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                           const mbx_beam_extents *beam, const mbx_consts *k,
+                           const mbx_pair_plan *plan, int options, void *s) {
+    return forward_static<float>(d, in, out, beam, k, plan, options, (hipStream_t)s);
+}
+int mbx_forward_static_f64(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *out,
+                           const mbx_beam_extents *beam, const mbx_consts *k,
+                           const mbx_pair_plan *plan, int options, void *s) {
+    return forward_static<double>(d, in, out, beam, k, plan, options, (hipStream_t)s);
+}
 
 size_t mbx_forward_workspace_bytes(const mbx_dims *d, int is_double) {
     if (!d) return 0;

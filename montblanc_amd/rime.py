@@ -311,14 +311,17 @@ def pair_plan(arrays, pair_block=None, point_gemm=True):
 
 def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=None,
             outputs=None, keep_ekb=False, dims=None, stream=None, pair_block=None,
-            point_gemm=True, workspace=None):
+            point_gemm=True, workspace=None, static_scratch=False):
     """Fused forward on device tensors. Returns the outputs dict
     (model_vis, chi_sqrd_result, X2 [device float64 scalar], ekb_vis?).
     point_gemm=False keeps every source on the vector units (True: point
     sources, and from 40 antennas in float / 48 in float64 the gaussian and
     sersic sources, on the matrix cores where the layout allows);
     workspace (a Workspace) replaces the per-device one, for concurrent
-    calls on several streams."""
+    calls on several streams; static_scratch=True calls the entry point
+    whose workspace is the library's own device array
+    (mbx_forward_static_*, at most _lib.STATIC_SCRATCH_BYTES; calls that
+    overlap on one device race on it)."""
     dev = arrays['uvw'].device
     check_arrays(arrays, counts, dims=dims)
     out = outputs if outputs is not None else alloc_outputs(arrays, keep_ekb)
@@ -329,12 +332,18 @@ def forward(arrays, counts, residual=False, use_weight_vector=False, sigma_sqrd=
     ins = _lib.mbx_inputs(**{n: _raw(arrays.get(n)) for n in _lib.INPUT_NAMES})
     outs = _lib.mbx_outputs(_raw(out.get('model_vis')), _raw(out.get('chi_sqrd_result')),
                             _raw(out.get('X2')), _raw(out.get('ekb_vis')))
-    lib = _lib.load()
-    nb = lib.mbx_forward_workspace_bytes(ctypes.byref(d), int(is_double(arrays)))
-    ws = (workspace or _workspace(dev)).get('forward', nb)
     b = beam_extents(arrays)
     k = consts(arrays, sigma_sqrd)
     h = stream_of(dev) if stream is None else ctypes.c_void_p(stream)
+    if static_scratch:
+        _lib.call('mbx_forward_static_' + suffix(arrays), ctypes.byref(d), ctypes.byref(ins),
+                  ctypes.byref(outs), ctypes.byref(b), ctypes.byref(k),
+                  ctypes.byref(plan[0]) if plan is not None else None,
+                  options(residual, use_weight_vector), h)
+        return out
+    lib = _lib.load()
+    nb = lib.mbx_forward_workspace_bytes(ctypes.byref(d), int(is_double(arrays)))
+    ws = (workspace or _workspace(dev)).get('forward', nb)
     _lib.call('mbx_forward_' + suffix(arrays), ctypes.byref(d), ctypes.byref(ins),
               ctypes.byref(outs), ctypes.byref(b), ctypes.byref(k),
               ctypes.byref(plan[0]) if plan is not None else None,

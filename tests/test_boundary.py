@@ -91,6 +91,19 @@ def test_validation_errors_without_launching():
     rc = lib.mbx_forward_f64(ctypes.byref(dims_for()), ctypes.byref(ins), ctypes.byref(outs),
                              ctypes.byref(b), ctypes.byref(k), None, 0, fake, 1 << 30, None)
     assert rc == -1
+    # the no-allocation forward rejects dimensions whose workspace exceeds
+    # its static array before touching a device: 256 channels of a
+    # 129 x 129 beam are 204 MB of beam planes in float32
+    big = dims_for()
+    big.nchan = _lib.mbx_dim(256, 256, 0, 256)
+    big.npolchan = _lib.mbx_dim(1024, 1024, 0, 1024)
+    for n in ('beam_lw', 'beam_mh'):
+        setattr(big, n, _lib.mbx_dim(129, 129, 0, 129))
+    assert lib.mbx_forward_workspace_bytes(ctypes.byref(big), 0) > _lib.STATIC_SCRATCH_BYTES
+    rc = lib.mbx_forward_static_f32(ctypes.byref(big), ctypes.byref(ins), ctypes.byref(outs),
+                                    ctypes.byref(b), ctypes.byref(k), None, 0, None)
+    assert rc == -4 and b'static scratch' in lib.mbx_last_error()
+    assert lib.mbx_forward_static_f64(None, None, None, None, None, None, 0, None) == -1
 
 
 def test_device_count_never_fails():
