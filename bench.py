@@ -19,6 +19,13 @@ same workload in one process; cpu_baseline_parallel the same slice with
 the coherency sum split over source ranges in one process per core (up to
 16).
 
+Default legs: the forward workload in float32 and float64, chi^2 + gradient
+(27 antennas in float64; the forward workload in float32), the 256-sample
+batched evaluation, the large single-model workload (64 antennas, 500
+times, 256 channels, 12000 sources: forward and forward + gradient, with
+the error against the float64 oracle on a slice), the attainable library
+GEMM rates and the CPU baselines.
+
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 """
 
@@ -42,8 +49,8 @@ def parse():
     p.add_argument('--dtypes', default='f32,f64')
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-gradient', action='store_true')
-    p.add_argument('--large', action='store_true',
-                   help='also run the large single-model workload (about a minute)')
+    p.add_argument('--no-large', action='store_true',
+                   help='skip the large single-model workload (about 40 s)')
     p.add_argument('--no-batched', action='store_true')
     p.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                    help='first run a plain copy kernel over MIB MiB (reads and writes '
@@ -133,6 +140,7 @@ def time_forward(w, dtype, steps, warmup):
     return dict(ms_per_step=wall * 1e3 / steps, ms_event_mean=ms_mean, ms_event_median=ms_median,
                 terms=n_terms, flops=flops, X2=x2,
                 terms_per_s=n_terms / (wall / steps),
+                terms_per_s_median=n_terms / (ms_median * 1e-3),
                 tflops=flops / (ms_mean * 1e-3) / 1e12)
 
 
@@ -164,25 +172,34 @@ def time_gradient(w, dtype, steps, warmup, name='gradient_27ant'):
         rime.forward(arrays, counts, outputs=out)
         rime.adjoint(arrays, counts, out['ekb_vis'], grads=grads)
 
-    res = {}
-    for name, fn in (('forward', fwd), ('forward_gradient', fwd_grad)):
+    res, med = {}, {}
+    stream = torch.cuda.current_stream()
+    for leg, fn in (('forward', fwd), ('forward_gradient', fwd_grad)):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for e0, e1 in ev:
+            e0.record(stream)
             fn()
+            e1.record(stream)
         torch.cuda.synchronize()
-        res[name] = (time.perf_counter() - t0) * 1e3 / steps
+        res[leg] = (time.perf_counter() - t0) * 1e3 / steps
+        med[leg] = sorted(e0.elapsed_time(e1) for e0, e1 in ev)[steps // 2]
     nbl = w['na'] * (w['na'] - 1) // 2
     n_terms = sum(counts) * w['ntime'] * nbl * w['nchan']
     del arrays, out, grads
     torch.cuda.empty_cache()
-    return {'workload': name, 'dtype': dtype, 'terms': n_terms,
+    return {'workload': name, 'dtype': dtype, 'terms': n_terms, 'steps': steps,
             'forward_ms': res['forward'], 'forward_gradient_ms': res['forward_gradient'],
+            'forward_ms_median': med['forward'],
+            'forward_gradient_ms_median': med['forward_gradient'],
             'forward_terms_per_s': n_terms / (res['forward'] * 1e-3),
             'gradient_terms_per_s': n_terms / (res['forward_gradient'] * 1e-3),
-            'gradient_over_forward': res['forward'] / res['forward_gradient']}
+            'gradient_over_forward': res['forward'] / res['forward_gradient'],
+            'gradient_over_forward_median': med['forward'] / med['forward_gradient']}
 
 
 def time_large(steps=1):
@@ -435,8 +452,14 @@ def main():
                                  'busy cycles, LDS bank conflicts 13% (rocprofv3 PMC, '
                                  'forward_64ant f32; k_shaped_mx counted before its scaled B '
                                  'operands were scheduled ahead of the products, 12.2 -> 11.3 ms)'},
+        'timing_note': 'value and ms_per_step: wall clock over the timed steps with the device '
+                       'synchronised on both sides; ms_event_median / terms_per_s_median: the '
+                       'median of per-step HIP events (the reference protocol: warm-up, then the '
+                       'median of the timed steps)',
         'per_dtype': {k: {'terms_per_s': v['terms_per_s'], 'ms_per_step': v['ms_per_step'],
-                          'ms_event_mean': v['ms_event_mean'], 'tflops': v['tflops'],
+                          'ms_event_mean': v['ms_event_mean'],
+                          'ms_event_median': v['ms_event_median'],
+                          'terms_per_s_median': v['terms_per_s_median'], 'tflops': v['tflops'],
                           'frac_of_peak': v['tflops'] / roofline.PEAK_TFLOPS[k], 'X2': v['X2']}
                       for k, v in results.items()},
     }
@@ -455,7 +478,7 @@ def main():
         line['gradient_f32'] = g32
     if not args.no_batched:
         line['batched'] = time_batched()
-    if args.large:
+    if not args.no_large:
         line['large'] = time_large()
         print('# large: %s' % line['large'], file=sys.stderr, flush=True)
     att = attainable_peaks()

@@ -65,16 +65,19 @@ static __host__ __device__ inline size_t gemm_x_offset(int na) {
     return ((sizeof(double) * (9 * size_t(na) + CH)) + 31) & ~size_t(31);
 }
 
-template <typename T>
+// Then the two source-record buffers, and in the shaped pass the shape
+// coordinates and kinds of both tiles.
+template <typename T, bool SHAPED>
 static inline size_t gemm_lds_bytes(int na, int st) {
     return gemm_x_offset(na) + 2 * size_t(st) * gemm_rows(na) * sizeof(XRow<T>) +
-           2 * 2 * size_t(na) * st * sizeof(T) + 2 * size_t(st) * sizeof(int);
+           2 * size_t(st) * sizeof(SrcRec<T>) +
+           (SHAPED ? 2 * 2 * size_t(na) * st * sizeof(T) + 2 * size_t(st) * sizeof(int) : 0);
 }
 
-template <typename T>
+template <typename T, bool SHAPED>
 static inline int gemm_st(int na, size_t budget) {
     int st = 32;
-    while (st > 4 && gemm_lds_bytes<T>(na, st) > budget) st -= 4;
+    while (st > 4 && gemm_lds_bytes<T, SHAPED>(na, st) > budget) st -= 4;
     return st;
 }
 
@@ -130,20 +133,15 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     double *Lpa = Luvw + 3 * NA;
     double *Lant = Lpa + 2 * NA;
     double *Lch = Lant + 4 * NA;
-    // two source-tile buffers: Jones X[s][row] (four reals per row), the
-    // shaped tiles' shape coordinates XY[s][a] = (x, y) and source kinds
+    // two source-tile buffers: Jones X[s][row] (four reals per row); two
+    // source-record buffers; in the shaped pass the shape coordinates
+    // XY[s][a] = (x, y) and source kinds of both tiles
     T *Xbase = reinterpret_cast<T *>(smem_raw + gemm_x_offset(NA));
     const int xstride = 4 * st * rows;       // T per Jones buffer
-    T *XYbase = Xbase + 2 * xstride;
+    SrcRec<T> *Pbase = reinterpret_cast<SrcRec<T> *>(Xbase + 2 * xstride);
+    T *XYbase = reinterpret_cast<T *>(Pbase + 2 * st);
     const int xystride = 2 * NA * st;        // T per coordinate buffer
     int *Kbase = reinterpret_cast<int *>(XYbase + 2 * xystride);
-
-    tile_setup<T>(A, Luvw, Lpa, Lch, t, c, tid, kGemmThreads);
-    stage_antennas<T>(A, Lant, t, c, tid, kGemmThreads);
-    // Jones rows past 2 na read as zero: clear both buffers once (point
-    // and shaped tiles write rows < 2 na only)
-    for (int i = tid; i < 2 * xstride; i += kGemmThreads) Xbase[i] = T(0);
-    __syncthreads();
 
     const int np = dext(d.npsrc), ns = dext(d.nsrc);
     const int sbeg = SHAPED ? np : 0, send = SHAPED ? ns : np;
@@ -153,6 +151,28 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
         nst = min(st, send - s0);
     };
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // source records of tile it into buffer it & 1, one producer thread per
+    // source (they read no LDS: tiles 0 and 1 go before the set-up barrier)
+    auto records = [&](int it) {
+        const int ptid = tid - kGemmConsumers;
+        int s0, nst;
+        tile_range(it, s0, nst);
+        if (ptid >= 0 && ptid < nst)
+            source_rec<T>(A, t, c, s0 + ptid, sizeof(T) == 4 ? 1.2011224087864498 : 1.0,
+                          Pbase[(it & 1) * st + ptid]);
+    };
+    if (wave >= kGemmConsumers / 64) {
+        if (ntiles > 0) records(0);
+        if (ntiles > 1) records(1);
+    }
+
+    tile_setup<T>(A, Luvw, Lpa, Lch, t, c, tid, kGemmThreads);
+    stage_antennas<T>(A, Lant, t, c, tid, kGemmThreads);
+    // Jones rows past 2 na read as zero: clear both buffers once (point
+    // and shaped tiles write rows < 2 na only)
+    for (int i = tid; i < 2 * xstride; i += kGemmThreads) Xbase[i] = T(0);
+    __syncthreads();
+
     if (wave < kGemmConsumers / 64) {
         typedef typename Mfma<T>::acc_t acc_t;
         const int lane = tid & 63, i16 = lane & 15, g = lane >> 4;
@@ -284,79 +304,48 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
         }
     } else {
         const int ptid = tid - kGemmConsumers, pth = kGemmThreads - kGemmConsumers;
-        const uint64_t NGS = d.ngsrc.local_size, NSS = d.nssrc.local_size;
-        const int ng = dext(d.ngsrc);
-        auto jones = [&](int a, double l0, double m0, double nm1, const J4<T> &B) {
-            return staged_jones<T, kGemmTaps<T>()>(A, Luvw, Lpa, Lant, Lch, c, a, l0, m0, nm1, B);
-        };
-        auto source_b = [&](int s, double &l0, double &m0, double &nm1) {
-            return source_sqrtb<T>(A, Lch, t, s, l0, m0, nm1);
-        };
-        auto produce = [&](int it, int buf) {
+        // tile it into buffer it & 1 from its source records: one Jones per
+        // (source, antenna) item, items taken in order by the producer
+        // threads (consecutive threads take consecutive antennas of a
+        // source: conflict-free 16/32-byte row writes)
+        auto produce = [&](int it) {
+            const int buf = it & 1;
             int s0, nst;
             tile_range(it, s0, nst);
             const int nst4 = (nst + 3) & ~3;
-            if (!SHAPED) {
-                // X[s][row]; a group of consecutive threads shares a source
-                // and walks the antennas (conflict-free 16/32-byte writes)
-                XRow<T> *Xb = reinterpret_cast<XRow<T> *>(Xbase + buf * xstride);
-                const int gs = pth / nst;
-                const int si = ptid / gs, a0 = ptid % gs;
-                if (si < nst) {
-                    double l0, m0, nm1;
-                    const J4<T> B = source_b(s0 + si, l0, m0, nm1);
-                    for (int a = a0; a < NA; a += gs) {
-                        const J4<T> J = jones(a, l0, m0, nm1, B);
-                        Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
-                        Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
-                    }
-                }
-                // sources past the tile's end up to a multiple of four read as zero
-                for (int i = ptid; i < (nst4 - nst) * rows; i += pth)
-                    Xb[(nst + i / rows) * rows + i % rows] = XRow<T>{{T(0), T(0), T(0), T(0)}};
-            } else {
-                // X[s][row] as the point pass, XY[s][a] = (x, y): a group of
-                // consecutive threads shares a source and walks the antennas
-                XRow<T> *Xb = reinterpret_cast<XRow<T> *>(Xbase + buf * xstride);
-                T *XY = XYbase + buf * xystride;
-                int *kind = Kbase + buf * st;
-                const int gs = pth / nst;
-                const int si = ptid / gs, a0 = ptid % gs;
-                if (si < nst) {
-                    const int s = s0 + si;
-                    double l0, m0, nm1;
-                    const J4<T> B = source_b(s, l0, m0, nm1);
+            XRow<T> *Xb = reinterpret_cast<XRow<T> *>(Xbase + buf * xstride);
+            const SrcRec<T> *P = Pbase + buf * st;
+            T *XY = XYbase + buf * xystride;
+            int *kind = Kbase + buf * st;
+            for (int i = ptid; i < nst * NA; i += pth) {
+                const int si = i / NA, a = i - si * NA;
+                const SrcRec<T> &r = P[si];
+                const J4<T> J = staged_jones<T, kGemmTaps<T>()>(A, Luvw, Lpa, Lant, Lch, c, a, r.l0,
+                                                                r.m0, r.nm1, r.B);
+                Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
+                Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
+                if (SHAPED) {
                     // shape coordinates: linear in the antenna's (u, v)
-                    double cu, cv, du, dv;        // x = cu u + cv v, y = du u + dv v
-                    const bool gauss = s < np + ng;
-                    if (gauss) {
-                        const int gi = s - np;
-                        const double el = A.gauss[gi], em = A.gauss[gi + NGS], R = A.gauss[gi + 2 * NGS];
-                        const double k = A.gauss_scale * Lch[3] * (sizeof(T) == 4 ? 1.2011224087864498 : 1.0);
-                        cu = k * R * em; cv = -k * R * el; du = k * el; dv = k * em;
-                    } else {
-                        const int si2 = s - np - ng;
-                        const double e1 = A.sersic[si2], e2 = A.sersic[si2 + NSS];
-                        const double Rp = A.sersic[si2 + 2 * NSS] / (1.0 - e1 * e1 - e2 * e2);
-                        const double k = A.two_pi_over_c * Lch[3] * Rp;
-                        cu = k * (1.0 + e1); cv = k * e2; du = k * e2; dv = k * (1.0 - e1);
-                    }
-                    if (a0 == 0) kind[si] = gauss ? 1 : 2;
-                    for (int a = a0; a < NA; a += gs) {
-                        const J4<T> J = jones(a, l0, m0, nm1, B);
-                        Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
-                        Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
-                        const double u = Luvw[3 * a], v = Luvw[3 * a + 1];
-                        XY[(si * NA + a) * 2] = T(cu * u + cv * v);
-                        XY[(si * NA + a) * 2 + 1] = T(du * u + dv * v);
-                    }
+                    const double u = Luvw[3 * a], v = Luvw[3 * a + 1];
+                    XY[(si * NA + a) * 2] = T(r.cu * u + r.cv * v);
+                    XY[(si * NA + a) * 2 + 1] = T(r.du * u + r.dv * v);
+                    if (a == 0) kind[si] = int(r.kind);
                 }
             }
+            // point pass: sources past the tile's end up to a multiple of
+            // four read as zero
+            if (!SHAPED)
+                for (int i = ptid; i < (nst4 - nst) * rows; i += pth)
+                    Xb[(nst + i / rows) * rows + i % rows] = XRow<T>{{T(0), T(0), T(0), T(0)}};
         };
-        if (ntiles > 0) produce(0, 0);
+        // records of tile it + 2 go to buffer it & 1 while tile it + 1 is
+        // produced: tile it's records were last read before the barrier
+        // that ended the previous step
+        if (ntiles > 0) produce(0);
         __syncthreads();
         for (int it = 0; it < ntiles; ++it) {
-            if (it + 1 < ntiles) produce(it + 1, (it + 1) & 1);
+            if (it + 1 < ntiles) produce(it + 1);
+            if (it + 2 < ntiles) records(it + 2);
             __syncthreads();
         }
     }
@@ -409,7 +398,8 @@ static inline int mx_groups(int na, int sets) {
 __host__ __device__ inline size_t mx_buf_ants(int na, int st) { return size_t(st) * na + kMxMaxSets; }
 static inline size_t mx_lds_bytes(int na, int st) {
     return gemm_x_offset(na) + 2 * mx_buf_ants(na, st) * (4 * sizeof(XRow<float>) + sizeof(float2)) +
-           (kMxThreads / 64) * sizeof(double);   // per-wave chi^2 partials
+           (kMxThreads / 64) * sizeof(double) +  // per-wave chi^2 partials
+           2 * size_t(st) * sizeof(SrcRec<float>);   // source records of two tiles
 }
 
 struct MxPlan {
@@ -509,7 +499,29 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
     const int xstride = 4 * bufa;            // rows per buffer
     float2 *XYbase = reinterpret_cast<float2 *>(Xbase + 2 * xstride);
     double *Lpart = reinterpret_cast<double *>(XYbase + 2 * bufa);
+    SrcRec<float> *Pbase = reinterpret_cast<SrcRec<float> *>(Lpart + kMxThreads / 64);
     double term_sum = 0.0;
+
+    const int np = dext(d.npsrc), ns = dext(d.nsrc), ng = dext(d.ngsrc);
+    const int ntiles = (ns - np + st - 1) / st;
+    auto tile_range = [&](int it, int &s0, int &nst) {
+        s0 = np + it * st;
+        nst = min(st, ns - s0);
+    };
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // source records of tile it into buffer it & 1, one producer thread per
+    // source (no LDS reads: tiles 0 and 1 go before the set-up barrier)
+    auto records = [&](int it) {
+        const int ptid = tid - 64 * ncw;
+        int s0, nst;
+        tile_range(it, s0, nst);
+        if (ptid >= 0 && ptid < nst)
+            source_rec<float>(A, t, c, s0 + ptid, 1.2011224087864498, Pbase[(it & 1) * st + ptid]);
+    };
+    if (wave >= ncw) {
+        if (ntiles > 0) records(0);
+        if (ntiles > 1) records(1);
+    }
 
     tile_setup<float>(A, Luvw, Lpa, Lch, t, c, tid, kMxThreads);
     stage_antennas<float>(A, Lant, t, c, tid, kMxThreads);
@@ -521,13 +533,6 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
     }
     __syncthreads();
 
-    const int np = dext(d.npsrc), ns = dext(d.nsrc), ng = dext(d.ngsrc);
-    const int ntiles = (ns - np + st - 1) / st;
-    auto tile_range = [&](int it, int &s0, int &nst) {
-        s0 = np + it * st;
-        nst = min(st, ns - s0);
-    };
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (wave < ncw) {
         const int lane = tid & 63, blk = lane >> 2, i = lane & 3;
         // this block's group: quad Q, antennas p0 .. p0 + SETS - 1 (a
@@ -595,50 +600,39 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
         }
     } else {
         const int ptid = tid - 64 * ncw, pth = kMxThreads - 64 * ncw;
-        const uint64_t NGS = d.ngsrc.local_size, NSS = d.nssrc.local_size;
-        auto produce = [&](int it, int buf) {
+        // tile it into buffer it & 1 from its source records: one Jones per
+        // (source, antenna) item, items taken in order by the producer
+        // threads
+        auto produce = [&](int it) {
+            const int buf = it & 1;
             int s0, nst;
             tile_range(it, s0, nst);
             XRow<float> *Xb = Xbase + buf * xstride;
             float2 *XY = XYbase + buf * bufa;
-            // a group of consecutive threads shares a source and walks the antennas
-            const int gs = pth / nst;
-            const int si = ptid / gs, a0 = ptid % gs;
-            if (si >= nst) return;
-            const int s = s0 + si;
-            double l0, m0, nm1;
-            const J4<float> B = source_sqrtb<float>(A, Lch, t, s, l0, m0, nm1);
-            // shape coordinates: linear in the antenna's (u, v)
-            double cu, cv, du, dv;        // x = cu u + cv v, y = du u + dv v
-            const bool gauss = s < np + ng;
-            if (gauss) {
-                const int gi = s - np;
-                const double el = A.gauss[gi], em = A.gauss[gi + NGS], R = A.gauss[gi + 2 * NGS];
-                const double k = A.gauss_scale * Lch[3] * 1.2011224087864498;   // sqrt(log2 e): exp2
-                cu = k * R * em; cv = -k * R * el; du = k * el; dv = k * em;
-            } else {
-                const int si2 = s - np - ng;
-                const double e1 = A.sersic[si2], e2 = A.sersic[si2 + NSS];
-                const double Rp = A.sersic[si2 + 2 * NSS] / (1.0 - e1 * e1 - e2 * e2);
-                const double k = A.two_pi_over_c * Lch[3] * Rp;
-                cu = k * (1.0 + e1); cv = k * e2; du = k * e2; dv = k * (1.0 - e1);
-            }
-            for (int a = a0; a < NA; a += gs) {
-                const J4<float> J =
-                    staged_jones<float, kGemmTaps<float>()>(A, Luvw, Lpa, Lant, Lch, c, a, l0, m0, nm1, B);
+            const SrcRec<float> *P = Pbase + buf * st;
+            for (int i = ptid; i < nst * NA; i += pth) {
+                const int si = i / NA, a = i - si * NA;
+                const SrcRec<float> &sr = P[si];
+                const J4<float> J = staged_jones<float, kGemmTaps<float>()>(A, Luvw, Lpa, Lant, Lch, c, a,
+                                                                            sr.l0, sr.m0, sr.nm1, sr.B);
                 XRow<float> *r = Xb + 4 * (si * NA + a);
                 r[0] = XRow<float>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
                 r[1] = XRow<float>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
                 r[2] = XRow<float>{{J.v[0].im, J.v[1].im, -J.v[0].re, -J.v[1].re}};
                 r[3] = XRow<float>{{J.v[2].im, J.v[3].im, -J.v[2].re, -J.v[3].re}};
+                // shape coordinates: linear in the antenna's (u, v)
                 const double u = Luvw[3 * a], v = Luvw[3 * a + 1];
-                XY[si * NA + a] = float2{float(cu * u + cv * v), float(du * u + dv * v)};
+                XY[si * NA + a] = float2{float(sr.cu * u + sr.cv * v), float(sr.du * u + sr.dv * v)};
             }
         };
-        if (ntiles > 0) produce(0, 0);
+        // records of tile it + 2 go to buffer it & 1 while tile it + 1 is
+        // produced (tile it's records were last read before the barrier
+        // that ended the previous step)
+        if (ntiles > 0) produce(0);
         __syncthreads();
         for (int it = 0; it < ntiles; ++it) {
-            if (it + 1 < ntiles) produce(it + 1, (it + 1) & 1);
+            if (it + 1 < ntiles) produce(it + 1);
+            if (it + 2 < ntiles) records(it + 2);
             __syncthreads();
         }
     }
@@ -699,13 +693,13 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
     const int nR = G.rows / 16;
     G.ntp = nR * (nR + 1) / 2;
     static size_t budget = gemm_lds_budget();
-    G.st = gemm_st<T>(na, budget);
     const uint64_t nwg = gemm_workgroups(d);
     RimeArgs<T> Ag = A;
     Ag.ct = 1;
-    const size_t lds = gemm_lds_bytes<T>(na, G.st);
     const int np = dext(d.npsrc), ns = dext(d.nsrc);
     if (np > 0) {
+        G.st = gemm_st<T, false>(na, budget);
+        const size_t lds = gemm_lds_bytes<T, false>(na, G.st);
         constexpr int cw = gemm_consumers<false>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
         if (G.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
@@ -724,6 +718,8 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
         constexpr int cw = gemm_consumers<true>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
         if (G.tpw > gemm_max_tp<true>()) return set_error(MBX_E_UNSUPPORTED, "shaped gemm: %d antennas", na);
+        G.st = gemm_st<T, true>(na, budget);
+        const size_t lds = gemm_lds_bytes<T, true>(na, G.st);
         hipLaunchKernelGGL((k_point_gemm<T, true>), dim3(unsigned(nwg)), dim3(gemm_threads<T, true>()),
                            lds, stream, Ag, G);
         return hip_status(hipGetLastError(), "shaped gemm launch");

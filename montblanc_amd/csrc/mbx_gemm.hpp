@@ -58,6 +58,55 @@ __device__ __forceinline__ J4<T> source_sqrtb(const RimeArgs<T> &A, const double
     return brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
 }
 
+// Per-source record of a source tile, computed once per source (by one
+// thread, a tile ahead) and read by every producer thread that builds one
+// of the source's per-antenna Jones: l, m, n - 1, the shape-coordinate
+// coefficients (x = cu u + cv v, y = du u + dv v; zero for point sources)
+// and the kind (0 point, 1 gaussian, 2 sersic) in float64, then sqrt(B) of
+// the workgroup's channel c.
+template <typename T> struct alignas(16) SrcRec {
+    double l0, m0, nm1, cu, cv, du, dv, kind;
+    J4<T> B;
+};
+
+// The record of source s at time t and channel c. gauss_pre scales the
+// gaussian coordinates (sqrt(log2 e) where exp2 evaluates the factor).
+// Reads the channel's frequencies from global memory (usable before the
+// channel record is staged in LDS); the arithmetic is that of
+// source_sqrtb with tile_setup's channel record.
+template <typename T>
+__device__ __forceinline__ void source_rec(const RimeArgs<T> &A, int t, int c, int s,
+                                           double gauss_pre, SrcRec<T> &r) {
+    const mbx_dims &d = A.d;
+    const uint64_t NT = d.ntime.local_size;
+    const int np = dext(d.npsrc), ng = dext(d.ngsrc);
+    const uint64_t NGS = d.ngsrc.local_size, NSS = d.nssrc.local_size;
+    const double f = A.freq[c];
+    const double lf = log(f / double(A.ref_freq[c]));
+    const double l0 = A.lm[2 * uint64_t(s)], m0 = A.lm[2 * uint64_t(s) + 1];
+    r.l0 = l0; r.m0 = m0;
+    r.nm1 = sqrt(1.0 - l0 * l0 - m0 * m0) - 1.0;
+    const T *S = A.stokes + (uint64_t(s) * NT + t) * 4;
+    const T power = exp_t<T>(double(A.alpha[uint64_t(s) * NT + t]) * lf);
+    r.B = brightness_sqrt_scaled<T>(S[0], S[1], S[2], S[3], power);
+    r.cu = r.cv = r.du = r.dv = 0.0;
+    r.kind = 0.0;
+    if (s >= np && s < np + ng) {
+        const int gi = s - np;
+        const double el = A.gauss[gi], em = A.gauss[gi + NGS], R = A.gauss[gi + 2 * NGS];
+        const double k = A.gauss_scale * f * gauss_pre;
+        r.cu = k * R * em; r.cv = -k * R * el; r.du = k * el; r.dv = k * em;
+        r.kind = 1.0;
+    } else if (s >= np + ng) {
+        const int si = s - np - ng;
+        const double e1 = A.sersic[si], e2 = A.sersic[si + NSS];
+        const double Rp = A.sersic[si + 2 * NSS] / (1.0 - e1 * e1 - e2 * e2);
+        const double k = A.two_pi_over_c * f * Rp;
+        r.cu = k * (1.0 + e1); r.cv = k * e2; r.du = k * e2; r.dv = k * (1.0 - e1);
+        r.kind = 2.0;
+    }
+}
+
 // J = E K sqrt(B) of one (source, antenna a) for the workgroup's channel c
 // from LDS-staged per-antenna records: uvw [3], parallactic sin/cos [2],
 // pointing error and scaling [4], channel record Lch.
