@@ -179,8 +179,9 @@ class ChunkedRimeSolver(object):
 
     Full-size arrays live on the host (transfer_<name>/retrieve_<name>
     work on them); solve() runs the fused forward chunk by chunk over time
-    (and channels, and source batches through the nsrc extents when the
-    plan asks for them), writing model visibilities back to the host array
+    and channels (the fused forward's ladder reduces those two: its
+    resident arrays do not grow with the source count the way the staged
+    pipeline's per-source scratch does), writing model visibilities back to the host array
     (overwritten, or accumulated with vis_write_mode='sum') and summing
     X2 over chunks -- the host side of the reference's v5 solve loop
     (CompositeRimeSolver.py:793-1012, 1027-1215), on one device.
@@ -279,6 +280,10 @@ class ChunkedRimeSolver(object):
         x2 = torch.zeros((), dtype=torch.float64, device=dev)
         sigma = float(h.props['sigma_sqrd'])
 
+        sfx = 'f32' if self.ft == np.float32 else 'f64'
+        na = h.dims['na']
+        plans = {}     # launch plans by antenna layout, built from the host arrays
+
         def stage(job):
             # chunk inputs: host slice -> pinned buffer -> device, on the
             # copy stream; an event marks them ready
@@ -286,6 +291,15 @@ class ChunkedRimeSolver(object):
             a = h.slice(t0, t1, c0, c1, skip=STATIC_ARRAYS)
             arrays = dict(self._static)
             with torch.cuda.stream(copy):
+                # the launch plan from the host layout (the forward would
+                # otherwise copy the antennas back from the device, blocking
+                # the pipeline), shared by chunks of the same layout
+                a1 = _device_ready('antenna1', a['antenna1'])
+                a2 = _device_ready('antenna2', a['antenna2'])
+                lk = (a1.shape, a1.tobytes(), a2.tobytes())
+                if lk not in plans:
+                    plans[lk] = rime.plan_entry(a1, a2, na, dev, sfx)
+                arrays['_pair_plan'] = plans[lk]
                 for k, v in a.items():
                     if not isinstance(v, np.ndarray):
                         arrays[k] = v

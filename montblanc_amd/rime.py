@@ -289,6 +289,21 @@ def build_plan(antenna1, antenna2, na, device, form, point_gemm=True):
     return (_lib.mbx_pair_plan(blocks, nblocks, pb, qb, pw, bmap, 1 if bmap else 0), keep)
 
 
+def _plan_key(sfx, pair_block, point_gemm):
+    if pair_block is None:
+        pair_block = DEFAULT_PAIR_BLOCK[sfx]
+    return ('_pair_plan', pair_block if pair_block == 'generic' else tuple(pair_block),
+            bool(point_gemm)), pair_block
+
+
+def plan_entry(antenna1, antenna2, na, device, sfx, pair_block=None, point_gemm=True):
+    """The '_pair_plan' cache entry of a forward call, built from HOST
+    antenna arrays (no device-to-host copy): callers that hold the host
+    layout (the chunked solver) put it in the arrays dict beforehand."""
+    key, form = _plan_key(sfx, pair_block, point_gemm)
+    return (key, build_plan(np.asarray(antenna1), np.asarray(antenna2), na, device, form, point_gemm))
+
+
 def pair_plan(arrays, pair_block=None, point_gemm=True):
     """Device launch plan for the antenna layout of ``arrays`` (cached in
     the dict under '_pair_plan'); None selects the generic kernel.
@@ -296,15 +311,12 @@ def pair_plan(arrays, pair_block=None, point_gemm=True):
     pair_block: (pb, qb[, producer_waves]), 'generic', or None for the
     dtype default.
     """
-    if pair_block is None:
-        pair_block = DEFAULT_PAIR_BLOCK[suffix(arrays)]
-    key = ('_pair_plan', pair_block if pair_block == 'generic' else tuple(pair_block),
-           bool(point_gemm))
+    key, form = _plan_key(suffix(arrays), pair_block, point_gemm)
     cached = arrays.get('_pair_plan')
     if cached is not None and cached[0] == key:
         return cached[1]
     plan = build_plan(arrays['antenna1'].cpu().numpy(), arrays['antenna2'].cpu().numpy(),
-                      arrays['uvw'].shape[1], arrays['uvw'].device, pair_block, point_gemm)
+                      arrays['uvw'].shape[1], arrays['uvw'].device, form, point_gemm)
     arrays['_pair_plan'] = (key, plan)
     return plan
 
