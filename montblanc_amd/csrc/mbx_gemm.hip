@@ -317,8 +317,12 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
             const SrcRec<T> *P = Pbase + buf * st;
             T *XY = XYbase + buf * xystride;
             int *kind = Kbase + buf * st;
-            for (int i = ptid; i < nst * NA; i += pth) {
-                const int si = i / NA, a = i - si * NA;
+            // item i = si NA + a; the stride pth advances (si, a) by (dsi, da)
+            const int dsi = pth / NA, da = pth - dsi * NA;
+            int si = ptid / NA, a = ptid - si * NA;
+            for (; si < nst; si += dsi, a += da) {
+                if (a >= NA) { a -= NA; ++si; }
+                if (si >= nst) break;
                 const SrcRec<T> &r = P[si];
                 const J4<T> J = staged_jones<T, kGemmTaps<T>()>(A, Luvw, Lpa, Lant, Lch, c, a, r.l0,
                                                                 r.m0, r.nm1, r.B);
@@ -610,8 +614,12 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
             XRow<float> *Xb = Xbase + buf * xstride;
             float2 *XY = XYbase + buf * bufa;
             const SrcRec<float> *P = Pbase + buf * st;
-            for (int i = ptid; i < nst * NA; i += pth) {
-                const int si = i / NA, a = i - si * NA;
+            // item i = si NA + a; the stride pth advances (si, a) by (dsi, da)
+            const int dsi = pth / NA, da = pth - dsi * NA;
+            int si = ptid / NA, a = ptid - si * NA;
+            for (; si < nst; si += dsi, a += da) {
+                if (a >= NA) { a -= NA; ++si; }
+                if (si >= nst) break;
                 const SrcRec<float> &sr = P[si];
                 const J4<float> J = staged_jones<float, kGemmTaps<float>()>(A, Luvw, Lpa, Lant, Lch, c, a,
                                                                             sr.l0, sr.m0, sr.nm1, sr.B);
