@@ -355,8 +355,10 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             // one (source, antenna p) per thread: st * NA <= threads. The
             // shape-factor adjoint of a pair is the same in both orders
             // (f and its derivatives are even in the pair's (u, v), and
-            // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): it is formed for
-            // q > p only and counted twice.
+            // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): every thread sums it
+            // over all its q and the combination takes half, which keeps the
+            // pair loop free of a branch that diverges inside every wave
+            // (a wave holds several antennas p).
             const int part = min(tid / (nst * NA), kSplit - 1);
             const int r = tid - part * nst * NA;
             const bool mine = r < nst * NA;
@@ -421,10 +423,10 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         const T dx = xy.x - xp, dy = xy.y - yp;
                         const T f = gauss_of_scaled(fma(dx, dx, dy * dy));
                         accumulate(f, Y);
-                        if (q > p) moments(f * jdot_re(Jp, Y), dx, dy);
+                        moments(f * jdot_re(Jp, Y), dx, dy);
                         advance(q);
                     }
-                    const T c4 = T(-4) * k / T(adj_gauss_pre<T>());
+                    const T c4 = T(-2) * k / T(adj_gauss_pre<T>());
                     sh0 = c4 * (m_yu - Rg * m_xv);
                     sh1 = c4 * fma(Rg, m_xu, m_yv);
                     sh2 = c4 * (em * m_xu - el * m_xv);
@@ -444,16 +446,14 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         T id;
                         const T f = sersic_factor(r2, id);
                         accumulate(f, Y);
-                        if (q > p) {
-                            const T tq = f * id * jdot_re(Jp, Y);
-                            moments(tq, dx, dy);
-                            m_r = fma(tq, r2, m_r);
-                        }
+                        const T tq = f * id * jdot_re(Jp, Y);
+                        moments(tq, dx, dy);
+                        m_r = fma(tq, r2, m_r);
                         advance(q);
                     }
-                    sh0 = T(-6) * fma(kR, m_xu - m_yv, a1 * m_r);
-                    sh1 = T(-6) * fma(kR, m_xv + m_yu, a2 * m_r);
-                    sh2 = T(-6) * kd * ((T(1) + e1) * m_xu + e2 * (m_xv + m_yu) + (T(1) - e1) * m_yv);
+                    sh0 = T(-3) * fma(kR, m_xu - m_yv, a1 * m_r);
+                    sh1 = T(-3) * fma(kR, m_xv + m_yu, a2 * m_r);
+                    sh2 = T(-3) * kd * ((T(1) + e1) * m_xu + e2 * (m_xv + m_yu) + (T(1) - e1) * m_yv);
                 }
             }
             __syncthreads();                       // every X read done
