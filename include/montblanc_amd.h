@@ -308,19 +308,27 @@ typedef struct mbx_batch {
                                       permutation; largest first balances)    */
 } mbx_batch;
 
-/* Launch shape; zero fields take the defaults (also what the autotuner
- * in montblanc_amd/autotune.py varies). */
+/* Strategy and launch shape; zero fields take the defaults (also what
+ * the autotuner in montblanc_amd/autotune.py varies).
+ * strategy 0: one persistent launch on the vector units, work items of
+ *   (sample, time, channel tile) taken from a queue.
+ * strategy 1 (float, at most 64 antennas, baseline_map required): the
+ *   matrix-core passes of the single-model forward and adjoint, each pass
+ *   one launch over a group of samples_per_group samples (one grid of
+ *   (time, channel) workgroups per sample, samples in `order`); the
+ *   workspace then holds two visibility arrays per sample of a group. */
 typedef struct mbx_batch_tuning {
-    int32_t workgroups;          /* persistent workgroups (default 2 per CU)  */
-    int32_t channels_per_item;   /* channels per work item (default 1)        */
-    int32_t sources_per_tile;    /* forward source tile (default: LDS bound)  */
-    int32_t reserved;
+    int32_t workgroups;          /* 0: persistent workgroups (default 2 per CU) */
+    int32_t channels_per_item;   /* 0: channels per work item (default 1)       */
+    int32_t sources_per_tile;    /* 0: forward source tile (default: LDS bound) */
+    int32_t strategy;            /* 0 or 1, above                               */
+    int32_t samples_per_group;   /* 1: samples per group of launches (> 0)      */
 } mbx_batch_tuning;
 
 /* X2[k] = chi^2 of sample k (device float64 [nsamples]). With grads
  * (lm/stokes/alpha/gauss_shape/sersic_shape shaped like the concatenated
  * inputs, float64; G_term must be NULL) the parameter gradients of every
- * sample's chi^2 are produced in the same launch; baseline_map as for
+ * sample's chi^2 are produced in the same call; baseline_map as for
  * mbx_adjoint_*. Model visibilities are not written. */
 size_t mbx_batched_workspace_bytes(const mbx_dims *d, int is_double, const mbx_batch_tuning *tune);
 int mbx_batched_f32(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *batch, double *X2,

@@ -96,7 +96,7 @@ class mbx_batch(ctypes.Structure):
 
 class mbx_batch_tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ('workgroups', 'channels_per_item',
-                                              'sources_per_tile', 'reserved')]
+                                              'sources_per_tile', 'strategy', 'samples_per_group')]
 
 
 BL_REVERSED = 1 << 30

@@ -120,8 +120,16 @@ def batched_tuning(batch, gradients=False, ntime_probe=2):
         for cpi in (1, 2):
             for st in (0, 8):
                 cands.append({'workgroups': wg, 'channels_per_item': cpi, 'sources_per_tile': st})
-    # or every sample through the single-model matrix-core kernels, as one
-    # captured graph
+    # the matrix-core passes, each one launch over groups of samples (float,
+    # one baseline per antenna pair at every time; the library rejects
+    # more than 64 antennas), or every sample through the single-model
+    # matrix-core kernels, as one captured graph
+    if not rime.is_double(a):
+        try:
+            rime.baseline_map(a)
+            cands.append({'strategy': 'matrix'})
+        except ValueError:
+            pass
     cands.append({'strategy': 'per_sample'})
 
     p1 = min(ntime, ntime_probe)

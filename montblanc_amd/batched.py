@@ -2,10 +2,13 @@
 
 ``Batch`` holds one set of visibility-space inputs (uvw, antennas,
 frequencies, beam, gains, flags, weights, observed visibilities) on the
-device and the concatenated source lists of K samples; ``evaluate()`` runs
-the library's single persistent launch over all samples and returns the
+device and the concatenated source lists of K samples; ``evaluate()`` returns the
 per-sample chi^2 and, on request, the gradients of every sample's chi^2
-with respect to its own source parameters. No reference counterpart.
+with respect to its own source parameters, by one of three strategies:
+the library's matrix-core passes each launched once over a group of
+samples, its single persistent launch on the vector units, or every
+sample through the single-model entry points in a captured graph. No
+reference counterpart.
 """
 
 import ctypes
@@ -63,6 +66,7 @@ class Batch(object):
         self._grads = None
         self._ws = None
         self._per_sample = {}
+        self._matrix_group = None
 
     @property
     def nsamples(self):
@@ -212,12 +216,35 @@ def drop_probes(batch):
         del batch._per_sample[k]
 
 
-def tuning_struct(tune):
+# fraction of the free device memory the matrix strategy's per-sample
+# visibility buffers may take
+MATRIX_MEMORY_FRACTION = 0.4
+
+
+def matrix_group(batch):
+    """Samples per group of launches for the matrix strategy: as many as
+    fit MATRIX_MEMORY_FRACTION of the free device memory with two
+    visibility arrays (running / pre-gain sums and Vb) per sample. Decided
+    once per batch (so later calls, graph capture included, query no
+    device memory)."""
+    if batch._matrix_group is None:
+        a = batch.arrays
+        ntime, nbl = a['antenna1'].shape
+        nchan = a['frequency'].shape[0]
+        per = 2 * ntime * nbl * nchan * 4 * rime.complex_dtype(a).itemsize
+        free = torch.cuda.mem_get_info(batch.device)[0]
+        batch._matrix_group = max(1, min(batch.nsamples, int(MATRIX_MEMORY_FRACTION * free // per)))
+    return batch._matrix_group
+
+
+def tuning_struct(tune, batch=None):
     if tune is None or tune.get('strategy') == 'per_sample':
         return None
-    t = _lib.mbx_batch_tuning(int(tune.get('workgroups', 0)), int(tune.get('channels_per_item', 0)),
-                              int(tune.get('sources_per_tile', 0)), 0)
-    return t
+    if tune.get('strategy') == 'matrix':
+        g = int(tune.get('samples_per_group', 0)) or matrix_group(batch)
+        return _lib.mbx_batch_tuning(0, 0, 0, 1, g)
+    return _lib.mbx_batch_tuning(int(tune.get('workgroups', 0)), int(tune.get('channels_per_item', 0)),
+                                 int(tune.get('sources_per_tile', 0)), 0, 0)
 
 
 def evaluate(batch, gradients=False, use_weight_vector=False, sigma_sqrd=None, tune='auto',
@@ -226,10 +253,13 @@ def evaluate(batch, gradients=False, use_weight_vector=False, sigma_sqrd=None, t
     a dict of float64 gradient tensors over the concatenated sources.
 
     tune: a dict (workgroups, channels_per_item, sources_per_tile) for the
-    persistent launch, {'strategy': 'per_sample'} for the per-sample graph
-    (which runs on the current stream; ``stream`` applies to the persistent
-    launch), None for the library defaults, or 'auto' for the cached
-    autotuner choice.
+    persistent launch; {'strategy': 'matrix'} (float, at most 64
+    antennas; optional 'samples_per_group', default: what fits
+    MATRIX_MEMORY_FRACTION of the free memory) for the matrix-core passes,
+    each one launch over a group of samples; {'strategy': 'per_sample'} for
+    the per-sample graph (which runs on the current stream; ``stream``
+    applies to the library launches); None for the library defaults, or
+    'auto' for the cached autotuner choice.
     """
     if tune == 'auto':
         from . import autotune
@@ -248,8 +278,9 @@ def evaluate(batch, gradients=False, use_weight_vector=False, sigma_sqrd=None, t
         return batch.X2
     a = batch.arrays
     d = batch.dims() if dims is None else dims
-    ts = tuning_struct(tune)
+    ts = tuning_struct(tune, batch)
     tref = ctypes.byref(ts) if ts is not None else None
+    matrix = ts is not None and ts.strategy == 1
     lib = _lib.load()
     nb = lib.mbx_batched_workspace_bytes(ctypes.byref(d), int(rime.is_double(a)), tref)
     if nb == 0:
@@ -258,11 +289,10 @@ def evaluate(batch, gradients=False, use_weight_vector=False, sigma_sqrd=None, t
         batch._ws = torch.empty(nb, dtype=torch.uint8, device=batch.device)
     ins = _lib.mbx_inputs(**{n: rime._raw(a.get(n)) for n in _lib.INPUT_NAMES})
     g = None
-    bmap = None
+    bmap = rime.baseline_map(a) if (gradients or matrix) else None
     if gradients:
         gb = batch.grads_buffers()
         g = _lib.mbx_grads(**{n: rime._raw(gb.get(n)) for n in _lib.GRAD_NAMES})
-        bmap = rime.baseline_map(a)
     b = rime.beam_extents(a)
     k = rime.consts(a, sigma_sqrd)
     h = rime.stream_of(batch.device) if stream is None else ctypes.c_void_p(stream)

@@ -104,7 +104,7 @@ static __device__ __forceinline__ double sersic_factor(double r2, double &id) {
 template <typename T, bool WIDE> constexpr int adjg_threads() { return WIDE ? 1024 : kAdjThreads; }
 template <typename T, bool WIDE> constexpr int adjg_split() { return WIDE ? 2 : 1; }
 
-template <typename T, bool SHAPED, bool WIDE = false>
+template <typename T, bool SHAPED, bool WIDE = false, bool BATCHED = false>
 __global__ void __launch_bounds__((adjg_threads<T, WIDE>()))
 k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     static_assert(SHAPED || !WIDE, "the wide layout is for the shaped pass");
@@ -112,6 +112,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     constexpr int kThreads = adjg_threads<T, WIDE>();
     constexpr int kSplit = adjg_split<T, WIDE>();
     extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
     const RimeArgs<T> &R = A.R;
     const mbx_dims &d = R.d;
     const int NA = d.na.local_size;
@@ -120,7 +122,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     const int tid = threadIdx.x;
     // XCD order of kChanGroup (mbx_tile.hpp)
     int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) return;
     const int rows = S.rows, st = S.st;
 
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -592,12 +594,14 @@ struct AdjPointLds {
     }
 };
 
-template <typename T>
+template <typename T, bool BATCHED = false>
 __global__ void __launch_bounds__(AdjPoint<T>::kThreads)
 k_adj_point(AdjArgs<T> A, AdjPointShape S) {
     typedef AdjPoint<T> C;
     typedef typename Mfma<T>::acc_t acc_t;
     extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
     const RimeArgs<T> &R = A.R;
     const mbx_dims &d = R.d;
     const int NA = d.na.local_size;
@@ -605,7 +609,7 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan), nb_ext = dext(d.nbl);
     const int tid = threadIdx.x;
     int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) return;
     const int rows = S.rows, st = S.st, xs = S.xs, nth = C::kProducers + 64 * S.nrb;
     const AdjPointLds<T> L(NA, S);
 
@@ -866,23 +870,21 @@ bool adjoint_gemm_supported(const mbx_dims &d) {
     return adjg_shape<T>(d, S, lds, false) && adjg_shape<T>(d, S, lds, true);
 }
 
-template <typename T>
-int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
-    const mbx_dims &d = A.R.d;
-    AdjGemmShape S;
-    size_t lds;
-    if (!adjg_shape<T>(d, S, lds, false)) return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: layout");
-    const unsigned nwg = 8 * tc_per_xcd(dext(d.ntime), dext(d.nchan));
-    AdjArgs<T> Ag = A;
-    Ag.R.ct = 1;
+// The point pass (if any point sources) and the shaped pass (if any
+// gaussian or sersic sources), nwg workgroups each.
+template <typename T, bool BATCHED>
+static int launch_adjoint_passes(const AdjArgs<T> &Ag, const AdjGemmShape &S, size_t lds, unsigned nwg,
+                                 hipStream_t stream) {
+    const mbx_dims &d = Ag.R.d;
     if (dext(d.npsrc) > 0) {
         AdjPointShape PS;
         size_t plds;
         if (adjp_shape<T>(d, PS, plds))
-            hipLaunchKernelGGL((k_adj_point<T>), dim3(nwg), dim3(AdjPoint<T>::kProducers + 64 * PS.nrb),
-                               plds, stream, Ag, PS);
+            hipLaunchKernelGGL((k_adj_point<T, BATCHED>), dim3(nwg),
+                               dim3(AdjPoint<T>::kProducers + 64 * PS.nrb), plds, stream, Ag, PS);
         else
-            hipLaunchKernelGGL((k_adj_gemm<T, false>), dim3(nwg), dim3(kAdjThreads), lds, stream, Ag, S);
+            hipLaunchKernelGGL((k_adj_gemm<T, false, false, BATCHED>), dim3(nwg), dim3(kAdjThreads), lds,
+                               stream, Ag, S);
         int r = hip_status(hipGetLastError(), "adjoint point gemm launch");
         if (r) return r;
     }
@@ -897,21 +899,39 @@ int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream) {
             constexpr int wth = adjg_threads<T, true>(), wsp = adjg_split<T, true>();
             wide = adjg_shape<T>(d, SW, wlds, true, wth, wsp);
             if (wide)
-                hipLaunchKernelGGL((k_adj_gemm<T, true, true>), dim3(nwg), dim3(wth), wlds, stream, Ag, SW);
+                hipLaunchKernelGGL((k_adj_gemm<T, true, true, BATCHED>), dim3(nwg), dim3(wth), wlds, stream,
+                                   Ag, SW);
         }
         if (!wide) {
             if (!adjg_shape<T>(d, SW, wlds, true))
                 return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: shaped layout");
-            hipLaunchKernelGGL((k_adj_gemm<T, true>), dim3(nwg), dim3(kAdjThreads), wlds, stream, Ag, SW);
+            hipLaunchKernelGGL((k_adj_gemm<T, true, false, BATCHED>), dim3(nwg), dim3(kAdjThreads), wlds,
+                               stream, Ag, SW);
         }
         return hip_status(hipGetLastError(), "adjoint shaped launch");
     }
     return MBX_OK;
 }
 
+template <typename T>
+int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream, unsigned nrep) {
+    const mbx_dims &d = A.R.d;
+    AdjGemmShape S;
+    size_t lds;
+    if (!adjg_shape<T>(d, S, lds, false)) return set_error(MBX_E_UNSUPPORTED, "adjoint gemm: layout");
+    // per sample: one workgroup per (time, channel) in the XCD order
+    const unsigned nwg = 8 * tc_per_xcd(dext(d.ntime), dext(d.nchan)) * nrep;
+    if (A.R.span.wgs && A.R.span.wgs * nrep != nwg)
+        return set_error(MBX_E_DIMS, "adjoint gemm: span of %u workgroups per sample", A.R.span.wgs);
+    AdjArgs<T> Ag = A;
+    Ag.R.ct = 1;
+    return A.R.span.wgs ? launch_adjoint_passes<T, true>(Ag, S, lds, nwg, stream)
+                        : launch_adjoint_passes<T, false>(Ag, S, lds, nwg, stream);
+}
+
 template bool adjoint_gemm_supported<float>(const mbx_dims &);
 template bool adjoint_gemm_supported<double>(const mbx_dims &);
-template int launch_adjoint_gemm<float>(const AdjArgs<float> &, hipStream_t);
-template int launch_adjoint_gemm<double>(const AdjArgs<double> &, hipStream_t);
+template int launch_adjoint_gemm<float>(const AdjArgs<float> &, hipStream_t, unsigned);
+template int launch_adjoint_gemm<double>(const AdjArgs<double> &, hipStream_t, unsigned);
 
 }  // namespace mbx

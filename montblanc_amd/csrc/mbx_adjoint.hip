@@ -26,7 +26,7 @@
 
 namespace mbx {
 template <typename T> bool adjoint_gemm_supported(const mbx_dims &d);
-template <typename T> int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream);
+template <typename T> int launch_adjoint_gemm(const AdjArgs<T> &A, hipStream_t stream, unsigned nrep = 1);
 }  // namespace mbx
 
 namespace mbx {
@@ -34,19 +34,21 @@ namespace mbx {
 // ---------------------------------------------------------------------
 // Visibility stage: Mb, Vb and the gain adjoint.
 
-template <typename T>
+template <typename T, bool BATCHED = false>
 __global__ void __launch_bounds__(256)
 k_adj_vis(AdjArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double *Gacc = reinterpret_cast<double *>(smem_raw);   // [NA][ct2][8]
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
     const RimeArgs<T> &R = A.R;
     const mbx_dims &d = R.d;
     const int NA = d.na.local_size;
     const uint64_t NC = d.nchan.local_size, NBL = d.nbl.local_size;
     const int ct2 = A.ct2;
     const int tid = threadIdx.x;
-    const int ctile = blockIdx.x % A.nctile2;
-    const int t = blockIdx.x / A.nctile2;
+    const int ctile = at.bx % A.nctile2;
+    const int t = at.bx / A.nctile2;
     const int c0 = ctile * ct2;
     const int nb_ext = dext(d.nbl), nc_ext = dext(d.nchan);
     const bool active = t < int(dext(d.ntime));
@@ -269,6 +271,39 @@ static size_t adjoint_ws_bytes(const mbx_dims &d) {
 
 template <typename T> constexpr int adj_stb() { return sizeof(T) == 4 ? 8 : 4; }
 
+// Channel tiling of k_adj_vis (gain adjoint sums of a tile in LDS); its
+// LDS bytes.
+template <typename T>
+static size_t adj_vis_tiling(const mbx_dims &d, AdjArgs<T> &A) {
+    const int na = d.na.local_size;
+    A.ct2 = std::max(1, std::min<int>(8, d.nchan.local_size));
+    while (A.ct2 > 1 && size_t(na) * A.ct2 * 8 * sizeof(double) > 48 * 1024) --A.ct2;
+    A.nctile2 = (d.nchan.local_size + A.ct2 - 1) / A.ct2;
+    return size_t(na) * A.ct2 * 8 * sizeof(double);
+}
+
+// Adjoint of nrep samples (A.R.span set with its slot0; A.ekb and A.vbar
+// the first slot's buffers; gradients zeroed by the caller): the
+// visibility stage and the matrix-core source passes, each pass one
+// launch over all nrep samples.
+template <typename T>
+int launch_adjoint_samples(AdjArgs<T> A, unsigned nrep, hipStream_t stream) {
+    const mbx_dims &d = A.R.d;
+    if (!adjoint_gemm_supported<T>(d))
+        return set_error(MBX_E_UNSUPPORTED, "batched adjoint: %u antennas", d.na.local_size);
+    if (A.g_G) return set_error(MBX_E_UNSUPPORTED, "batched adjoint: gain gradients are not produced");
+    const uint64_t NT = d.ntime.local_size;
+    const size_t lds1 = adj_vis_tiling(d, A);
+    A.R.span.wgs = unsigned(NT * A.nctile2);
+    hipLaunchKernelGGL((k_adj_vis<T, true>), dim3(unsigned(NT * A.nctile2 * nrep)), dim3(256), lds1, stream, A);
+    const int r = hip_status(hipGetLastError(), "batched adjoint vis launch");
+    if (r) return r;
+    A.R.span.wgs = 8 * tc_per_xcd(dext(d.ntime), dext(d.nchan));
+    return launch_adjoint_gemm<T>(A, stream, nrep);
+}
+template int launch_adjoint_samples<float>(AdjArgs<float>, unsigned, hipStream_t);
+template int launch_adjoint_samples<double>(AdjArgs<double>, unsigned, hipStream_t);
+
 template <typename T>
 static int adjoint(const mbx_dims *d, const mbx_inputs *in, const void *ekb, const int32_t *bmap,
                    const mbx_grads *gr, const mbx_beam_extents *b, const mbx_consts *k, int options,
@@ -311,10 +346,7 @@ static int adjoint(const mbx_dims *d, const mbx_inputs *in, const void *ekb, con
     A.gscale = gscale * ((options & MBX_APPLY_WEIGHTS) ? 1.0 : 1.0 / k->sigma_sqrd);
 
     // visibility stage
-    A.ct2 = std::max(1, std::min<int>(8, d->nchan.local_size));
-    while (A.ct2 > 1 && size_t(na) * A.ct2 * 8 * sizeof(double) > 48 * 1024) --A.ct2;
-    A.nctile2 = (d->nchan.local_size + A.ct2 - 1) / A.ct2;
-    size_t lds1 = size_t(na) * A.ct2 * 8 * sizeof(double);
+    const size_t lds1 = adj_vis_tiling(*d, A);
     hipLaunchKernelGGL(k_adj_vis<T>, dim3(unsigned(NT * A.nctile2)), dim3(256), lds1, stream, A);
     r = hip_status(hipGetLastError(), "adjoint vis launch");
     if (r) return r;

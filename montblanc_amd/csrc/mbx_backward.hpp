@@ -22,6 +22,25 @@ template <typename T> struct AdjArgs {
 
 constexpr int kBlMask = (1 << 29) - 1;
 
+// enter_sample (mbx_tile.hpp) for the adjoint: also the slot's pre-gain
+// sums and Vb, and the sample's rows / columns of the gradient arrays
+// (concatenated like the source arrays).
+template <typename T>
+__device__ __forceinline__ SampleAt enter_sample(AdjArgs<T> &A) {
+    const SampleAt at = enter_sample(A.R);
+    if (A.R.span.wgs) {
+        const uint64_t off = uint64_t(at.j) * A.R.span.vstride, NT = A.R.d.ntime.local_size;
+        A.ekb += off;
+        A.vbar += off;
+        if (A.g_lm) A.g_lm += 2 * uint64_t(at.src0);
+        if (A.g_stokes) A.g_stokes += uint64_t(at.src0) * NT * 4;
+        if (A.g_alpha) A.g_alpha += uint64_t(at.src0) * NT;
+        if (A.g_gauss) A.g_gauss += at.goff;
+        if (A.g_sersic) A.g_sersic += at.soff;
+    }
+    return at;
+}
+
 
 template <typename T>
 __device__ __forceinline__ J4<T> jherm(const J4<T> &A) {

@@ -21,8 +21,12 @@
 #include <algorithm>
 
 #include "mbx_backward.hpp"
+#include "mbx_gemm.hpp"
 
 namespace mbx {
+
+template <typename T> bool adjoint_gemm_supported(const mbx_dims &d);
+template <typename T> int launch_adjoint_samples(AdjArgs<T> A, unsigned nrep, hipStream_t stream);
 
 struct BatchDev {
     const int32_t *src_offset, *npsrc, *ngsrc, *nssrc, *gauss_offset, *sersic_offset, *order;
@@ -365,8 +369,127 @@ static int plan_batched(const mbx_dims &d, const mbx_batch_tuning *tune, BatPlan
     return MBX_OK;
 }
 
+// ---- strategy 1: the matrix-core passes over groups of samples ---------
+
+static uint64_t mx_vstride(const mbx_dims &d) {
+    return uint64_t(d.ntime.local_size) * d.nbl.local_size * d.nchan.local_size * 4;
+}
+
+static size_t mx_partial_bytes(const mbx_dims &d, const mbx_batch_tuning *tune) {
+    return (uint64_t(tune->samples_per_group) * gemm_workgroups(d) * sizeof(double) + 255) & ~size_t(255);
+}
+
+// Beam planes, the chi^2 partials of a group's samples, then per slot of a
+// group the running / pre-gain sums, then per slot Vb.
+template <typename T>
+static size_t batched_mx_ws_bytes(const mbx_dims &d, const mbx_batch_tuning *tune) {
+    if (sizeof(T) == 8) {
+        set_error(MBX_E_UNSUPPORTED, "batched: the matrix-core strategy is float only");
+        return 0;
+    }
+    if (tune->samples_per_group <= 0) {
+        set_error(MBX_E_DIMS, "batched: samples_per_group must be positive (got %d)", tune->samples_per_group);
+        return 0;
+    }
+    return beam_planes_bytes<T>(d) + mx_partial_bytes(d, tune) +
+           2 * uint64_t(tune->samples_per_group) * mx_vstride(d) * sizeof(cx<T>);
+}
+
+// X2[order[slot0 + j]] = scale x (the sum of slot j's nwg partials, in a
+// fixed order): one block per slot.
+__global__ void __launch_bounds__(256)
+k_chi2_slots(const double *__restrict__ partial, unsigned nwg, const int32_t *__restrict__ order,
+             double scale, double *__restrict__ X2) {
+    __shared__ double red[256];
+    const double *p = partial + uint64_t(blockIdx.x) * nwg;
+    double acc = 0.0;
+    for (unsigned i = threadIdx.x; i < nwg; i += 256) acc += p[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (unsigned w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) X2[order[blockIdx.x]] = red[0] * scale;
+}
+
+// Gradients (if any) are zeroed by the caller.
+template <typename T>
+static int batched_mx(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *bt, double *X2,
+                      const mbx_grads *gr, bool grads, const int32_t *bmap, const mbx_beam_extents *b,
+                      const mbx_consts *k, const mbx_batch_tuning *tune, int options, void *ws,
+                      size_t ws_bytes, hipStream_t stream) {
+    if constexpr (sizeof(T) == 8) {
+        (void)d; (void)in; (void)bt; (void)X2; (void)gr; (void)grads; (void)bmap; (void)b; (void)k;
+        (void)tune; (void)options; (void)ws; (void)ws_bytes; (void)stream;
+        return set_error(MBX_E_UNSUPPORTED, "batched: the matrix-core strategy is float only");
+    } else {
+        if (!bmap) return set_error(MBX_E_NULL, "batched: the matrix-core strategy needs the baseline map");
+        if (!batched_mx_supported(*d))
+            return set_error(MBX_E_UNSUPPORTED, "batched: the matrix-core passes do not fit %u antennas",
+                             d->na.local_size);
+        if (grads && !adjoint_gemm_supported<T>(*d))
+            return set_error(MBX_E_UNSUPPORTED, "batched: the matrix-core adjoint does not fit %u antennas",
+                             d->na.local_size);
+        if (tune->samples_per_group <= 0)
+            return set_error(MBX_E_DIMS, "batched: samples_per_group must be positive (got %d)",
+                             tune->samples_per_group);
+        const size_t need = batched_mx_ws_bytes<T>(*d, tune);
+        if (!ws || ws_bytes < need)
+            return set_error(MBX_E_WORKSPACE, "batched: workspace needs %zu bytes (got %zu)", need, ws_bytes);
+        if (dext(d->ntime) == 0 || dext(d->nbl) == 0 || dext(d->nchan) == 0)
+            return hip_status(hipGetLastError(), "batched (empty extents)");
+        const uint64_t nwg = gemm_workgroups(*d), vstride = mx_vstride(*d);
+        const unsigned G = unsigned(tune->samples_per_group);
+        char *base = static_cast<char *>(ws);
+        BeamCell<T> *beam = reinterpret_cast<BeamCell<T> *>(base);
+        double *partial = reinterpret_cast<double *>(base + beam_planes_bytes<T>(*d));
+        cx<T> *buf = reinterpret_cast<cx<T> *>(base + beam_planes_bytes<T>(*d) + mx_partial_bytes(*d, tune));
+        cx<T> *vbar = buf + uint64_t(G) * vstride;
+        launch_beam_planes<T>(*d, in->E_beam, in->frequency, b, beam, stream);
+
+        RimeArgs<T> A;
+        fill_args<T>(A, d, in, b, k, options);
+        A.beam = beam;
+        A.partial = partial;
+        A.ekb = grads ? buf : nullptr;      // pre-gain sums kept for the adjoint
+        A.span = SampleSpan{bt->src_offset, bt->npsrc, bt->ngsrc, bt->nssrc, bt->gauss_offset,
+                            bt->sersic_offset, bt->order, unsigned(nwg), 0, vstride};
+        const double scale = (options & MBX_APPLY_WEIGHTS) ? 1.0 : 1.0 / k->sigma_sqrd;
+        AdjArgs<T> J;
+        if (grads) {
+            J.R = A;
+            J.R.ekb = nullptr;
+            J.R.partial = nullptr;
+            J.ekb = buf;
+            J.vbar = vbar;
+            J.bmap = bmap;
+            J.g_lm = gr->lm; J.g_stokes = gr->stokes; J.g_alpha = gr->alpha;
+            J.g_gauss = gr->gauss_shape; J.g_sersic = gr->sersic_shape;
+            J.g_G = nullptr;
+            J.gscale = scale;
+        }
+        const bool any_point = dext(d->npsrc) > 0;
+        for (int slot0 = 0; slot0 < bt->nsamples; slot0 += int(G)) {
+            const unsigned nrep = unsigned(std::min<int>(int(G), bt->nsamples - slot0));
+            A.span.slot0 = slot0;
+            int r = launch_batched_mx(A, bmap, buf, any_point, nrep, stream);
+            if (r) return r;
+            hipLaunchKernelGGL(k_chi2_slots, dim3(nrep), dim3(256), 0, stream, partial, unsigned(nwg),
+                               bt->order + slot0, scale, X2);
+            if ((r = hip_status(hipGetLastError(), "batched chi2 launch"))) return r;
+            if (grads) {
+                J.R.span = A.span;
+                if ((r = launch_adjoint_samples<T>(J, nrep, stream))) return r;
+            }
+        }
+        return MBX_OK;
+    }
+}
+
 template <typename T>
 static size_t batched_ws_bytes(const mbx_dims &d, const mbx_batch_tuning *tune) {
+    if (tune && tune->strategy == 1) return batched_mx_ws_bytes<T>(d, tune);
     BatPlan p;
     if (plan_batched<T>(d, tune, p)) return 0;
     size_t beam = beam_planes_bytes<T>(d);
@@ -389,16 +512,23 @@ static int batched(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *bt,
     const bool grads = gr && (gr->lm || gr->stokes || gr->alpha || gr->gauss_shape || gr->sersic_shape);
     if (grads && !bmap) return set_error(MBX_E_NULL, "batched: gradients need the baseline map");
     if (gr && gr->G_term) return set_error(MBX_E_UNSUPPORTED, "batched: gain gradients are not produced");
+    if (tune && tune->strategy != 0 && tune->strategy != 1)
+        return set_error(MBX_E_UNSUPPORTED, "batched: unknown strategy %d", tune->strategy);
+    const bool mx = tune && tune->strategy == 1;
     BatPlan p;
-    if ((r = plan_batched<T>(*d, tune, p))) return r;
+    if (!mx && (r = plan_batched<T>(*d, tune, p))) return r;
+    if (mx && (sizeof(T) == 8 || !bmap || tune->samples_per_group <= 0))
+        return batched_mx<T>(d, in, bt, X2, gr, grads, bmap, b, k, tune, options, ws, ws_bytes, stream);
     size_t need = batched_ws_bytes<T>(*d, tune);
     if (!ws || ws_bytes < need)
         return set_error(MBX_E_WORKSPACE, "batched: workspace needs %zu bytes (got %zu)", need, ws_bytes);
     const uint64_t NT = d->ntime.local_size;
     const uint64_t ns = d->nsrc.local_size;
-    // zero X2 and the gradients (fill kernels: safe under stream capture)
+    // zero X2 and the gradients (fill kernels: safe under stream capture);
+    // the persistent launch's work counter sits in its workspace's last
+    // 256 bytes
     hipLaunchKernelGGL(k_fill_batch, dim3(1), dim3(256), 0, stream, X2, uint64_t(bt->nsamples),
-                       reinterpret_cast<int *>(static_cast<char *>(ws) + need - 256));
+                       mx ? (int *)nullptr : reinterpret_cast<int *>(static_cast<char *>(ws) + need - 256));
     if (grads) {
         zero_doubles(gr->lm, ns * 2, stream);
         zero_doubles(gr->stokes, ns * NT * 4, stream);
@@ -406,6 +536,7 @@ static int batched(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *bt,
         zero_doubles(gr->gauss_shape, uint64_t(d->ngsrc.local_size) * 3, stream);
         zero_doubles(gr->sersic_shape, uint64_t(d->nssrc.local_size) * 3, stream);
     }
+    if (mx) return batched_mx<T>(d, in, bt, X2, gr, grads, bmap, b, k, tune, options, ws, ws_bytes, stream);
     if (dext(d->ntime) == 0 || dext(d->nbl) == 0 || dext(d->nchan) == 0)
         return hip_status(hipGetLastError(), "batched (empty extents)");
     BeamCell<T> *beam = static_cast<BeamCell<T> *>(ws);

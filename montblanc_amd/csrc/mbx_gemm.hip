@@ -113,20 +113,26 @@ __device__ __forceinline__ double swap_lane_pairs(double v) {
 // SHAPED = false: the point sources [0, npsrc); true: the gaussian and
 // sersic sources [npsrc, nsrc). Separate instances keep each one's
 // registers to what it needs.
-template <typename T, bool SHAPED>
+template <typename T, bool SHAPED, bool BATCHED = false>
 __global__ void __launch_bounds__((gemm_threads<T, SHAPED>()))
 k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
     constexpr int kGemmThreads = gemm_threads<T, SHAPED>();
     constexpr int kGemmConsumers = gemm_consumers<SHAPED>();
     constexpr int kGemmMaxTp = gemm_max_tp<SHAPED>();
     extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
+    if constexpr (BATCHED) {
+        G.dst += uint64_t(at.j) * A.span.vstride;
+        if (G.vin) G.vin += uint64_t(at.j) * A.span.vstride;
+    }
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const int tid = threadIdx.x;
     // XCD order of kChanGroup (mbx_tile.hpp)
     int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) return;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) return;
 
     const int rows = G.rows, st = G.st;
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -476,17 +482,23 @@ __device__ __forceinline__ void mx_sources(const XRow<float> *Xp, const XRow<flo
 // epi = 0: the sums go to G.dst (plus G.vin) as in the point pass; epi =
 // 1: the epilogue is applied here (vis_out) and a chi^2 partial per
 // workgroup written to A.partial.
-template <int SETS>
+template <int SETS, bool BATCHED = false>
 __global__ void __launch_bounds__(kMxThreads)
 k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
     extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
+    if constexpr (BATCHED) {
+        G.dst += uint64_t(at.j) * A.span.vstride;
+        if (G.vin) G.vin += uint64_t(at.j) * A.span.vstride;
+    }
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
     const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const int tid = threadIdx.x;
     int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c)) {
-        if (epi && tid == 0) A.partial[blockIdx.x] = 0.0;   // padding workgroup
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) {
+        if (epi && tid == 0) A.partial[at.bx] = 0.0;   // padding workgroup
         return;
     }
 
@@ -644,7 +656,7 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
             __syncthreads();
         }
     }
-    if (epi) wg_partial(term_sum, Lpart, A.partial + blockIdx.x, tid, kMxThreads);
+    if (epi) wg_partial(term_sum, Lpart, A.partial + at.bx, tid, kMxThreads);
 }
 
 static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_t nwg, int epi,
@@ -733,6 +745,53 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
         return hip_status(hipGetLastError(), "shaped gemm launch");
     }
     return MBX_OK;
+}
+
+bool batched_mx_supported(const mbx_dims &d) {
+    MxPlan p;
+    return point_gemm_supported<float>(d) && mx_plan(d.na.local_size, gemm_lds_budget(), p);
+}
+
+int launch_batched_mx(const RimeArgs<float> &A, const int32_t *bmap, cx<float> *buf, bool any_point,
+                      unsigned nrep, hipStream_t stream) {
+    const mbx_dims &d = A.d;
+    const int na = d.na.local_size;
+    if (!batched_mx_supported(d)) return set_error(MBX_E_UNSUPPORTED, "batched matrix passes: %d antennas", na);
+    const uint64_t nwg = gemm_workgroups(d);
+    if (A.span.wgs != nwg) return set_error(MBX_E_DIMS, "batched matrix passes: span of %u workgroups, "
+                                            "the passes need %llu", A.span.wgs, (unsigned long long)nwg);
+    const dim3 grid{unsigned(nwg * nrep)};
+    GemmArgs<float> G;
+    G.bmap = bmap; G.dst = buf; G.vin = nullptr;
+    G.rows = gemm_rows(na);
+    const int nR = G.rows / 16;
+    G.ntp = nR * (nR + 1) / 2;
+    static size_t budget = gemm_lds_budget();
+    RimeArgs<float> Ag = A;
+    Ag.ct = 1;
+    if (any_point) {
+        G.st = gemm_st<float, false>(na, budget);
+        const size_t lds = gemm_lds_bytes<float, false>(na, G.st);
+        constexpr int cw = gemm_consumers<false>() / 64;
+        G.tpw = (G.ntp + cw - 1) / cw;
+        if (G.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
+        hipLaunchKernelGGL((k_point_gemm<float, false, true>), grid, dim3(gemm_threads<float, false>()), lds,
+                           stream, Ag, G);
+        const int r = hip_status(hipGetLastError(), "batched point gemm launch");
+        if (r) return r;
+        G.vin = buf;
+    }
+    MxPlan p;
+    mx_plan(na, budget, p);
+    G.st = p.st;
+    const dim3 block{unsigned(kMxThreads)};
+    switch (p.sets) {
+    case 1: hipLaunchKernelGGL((k_shaped_mx<1, true>), grid, block, p.lds, stream, Ag, G, p.waves, 1); break;
+    case 2: hipLaunchKernelGGL((k_shaped_mx<2, true>), grid, block, p.lds, stream, Ag, G, p.waves, 1); break;
+    case 3: hipLaunchKernelGGL((k_shaped_mx<3, true>), grid, block, p.lds, stream, Ag, G, p.waves, 1); break;
+    default: hipLaunchKernelGGL((k_shaped_mx<4, true>), grid, block, p.lds, stream, Ag, G, p.waves, 1); break;
+    }
+    return hip_status(hipGetLastError(), "batched shaped pass launch");
 }
 
 template bool point_gemm_supported<float>(const mbx_dims &);

@@ -173,4 +173,15 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
 // to the XCD order).
 uint64_t gemm_workgroups(const mbx_dims &d);
 
+// Forward of nrep samples in one launch per pass (float; A.span set with
+// wgs = gemm_workgroups(A.d)): the point sources of every sample into its
+// slot of buf (nrep slots of A.span.vstride complex values), then the
+// gaussian and sersic sources on top, with the epilogue: chi^2 partials
+// per sample, A's model / chi^2 term arrays where set, and the pre-gain
+// sums where A.ekb is set (to buf itself, slot by slot). any_point: some
+// sample has point sources.
+int launch_batched_mx(const RimeArgs<float> &A, const int32_t *bmap, cx<float> *buf, bool any_point,
+                      unsigned nrep, hipStream_t stream);
+bool batched_mx_supported(const mbx_dims &d);
+
 }  // namespace mbx

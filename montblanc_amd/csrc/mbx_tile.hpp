@@ -22,7 +22,23 @@ namespace mbx {
 // cells instead of eight taps of four correlations.
 template <typename T> struct alignas(4 * sizeof(T)) BeamCell { T v[12]; };
 
+// Several samples in one launch (batched evaluation): workgroups
+// [j wgs, (j + 1) wgs) belong to slot slot0 + j of the processing order,
+// sample k = order[slot0 + j]. Sample k's sources are rows
+// [src_offset[k], src_offset[k] + n_k) of lm / stokes / alpha and columns
+// from gauss_offset[k] / sersic_offset[k] of the shape arrays (whose row
+// stride stays the total column count); per-slot visibility buffers
+// (running sums, pre-gain sums, Vb) lie vstride complex values apart, and
+// the chi^2 partials of slot j from j wgs on.
+struct SampleSpan {
+    const int32_t *src_offset, *npsrc, *ngsrc, *nssrc, *gauss_offset, *sersic_offset, *order;
+    uint32_t wgs;       // workgroups per sample; 0: one model
+    int32_t slot0;
+    uint64_t vstride;
+};
+
 template <typename T> struct RimeArgs {
+    SampleSpan span;
     mbx_dims d;
     const T *uvw, *lm, *stokes, *alpha, *freq, *ref_freq, *pa, *pe, *ascale, *gauss, *sersic, *wv;
     const BeamCell<T> *beam;  // [nchan][beam_lw][beam_mh]
@@ -122,6 +138,57 @@ static inline void fill_args(RimeArgs<T> &A, const mbx_dims *d, const mbx_inputs
     A.per_xcd = 0;
     A.blocks = nullptr; A.nblocks = 0; A.pb = A.qb = 0;
     A.ct = A.st = A.nbchunk = A.bchunk = A.nctile = 1;
+    A.span = SampleSpan{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0u, 0, 0ull};
+}
+
+// The workgroup's sample in a launch over a SampleSpan, and its block
+// index within that sample's grid (blockIdx.x for one model).
+struct SampleAt {
+    uint32_t bx;        // block index within the sample's grid
+    int j;              // slot within the launch
+    int k;              // sample
+    int src0, goff, soff;
+};
+
+// Points A at the workgroup's sample: source pointers and source extents
+// become the sample's, the running / pre-gain sum buffers (vin, ekb) and
+// the chi^2 partials the slot's.
+template <typename T>
+__device__ __forceinline__ SampleAt enter_sample(RimeArgs<T> &A) {
+    SampleAt s{blockIdx.x, 0, 0, 0, 0, 0};
+    const SampleSpan &S = A.span;
+    if (S.wgs == 0) return s;
+    s.j = int(blockIdx.x / S.wgs);
+    s.bx = blockIdx.x - uint32_t(s.j) * S.wgs;
+    s.k = S.order[S.slot0 + s.j];
+    s.src0 = S.src_offset[s.k];
+    s.goff = S.gauss_offset[s.k];
+    s.soff = S.sersic_offset[s.k];
+    const int np = S.npsrc[s.k], ng = S.ngsrc[s.k], nss = S.nssrc[s.k];
+    const uint64_t NT = A.d.ntime.local_size;
+    A.lm += 2 * uint64_t(s.src0);
+    A.stokes += uint64_t(s.src0) * NT * 4;
+    A.alpha += uint64_t(s.src0) * NT;
+    if (A.gauss) A.gauss += s.goff;
+    if (A.sersic) A.sersic += s.soff;
+    A.d.npsrc.lower_extent = 0; A.d.npsrc.upper_extent = np;
+    A.d.ngsrc.lower_extent = 0; A.d.ngsrc.upper_extent = ng;
+    A.d.nssrc.lower_extent = 0; A.d.nssrc.upper_extent = nss;
+    A.d.nsrc.lower_extent = 0; A.d.nsrc.upper_extent = np + ng + nss;
+    A.d.nsrc.global_size = np + ng + nss;
+    const uint64_t off = uint64_t(s.j) * S.vstride;
+    if (A.ekb) A.ekb += off;
+    if (A.vin) A.vin += off;
+    if (A.partial) A.partial += uint64_t(s.j) * S.wgs;
+    return s;
+}
+
+// enter_sample in the BATCHED instances of a kernel; one model otherwise
+// (those instances carry no span code).
+template <bool BATCHED, typename Args>
+__device__ __forceinline__ SampleAt enter_sample_if(Args &A) {
+    if constexpr (BATCHED) return enter_sample(A);
+    else return SampleAt{blockIdx.x, 0, 0, 0, 0, 0};
 }
 
 // Workgroup order. The dispatcher deals workgroups to the eight XCDs
@@ -135,13 +202,17 @@ constexpr int kChanGroup = 4;
 
 __host__ __device__ inline uint32_t xcd_slots(uint64_t nwg) { return uint32_t((nwg + 7) / 8); }
 
-__device__ __forceinline__ uint64_t xcd_slot(uint32_t per_xcd) {
-    return uint64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+__device__ __forceinline__ uint64_t xcd_slot(uint32_t per_xcd, uint32_t bx = blockIdx.x) {
+    return uint64_t(bx % 8) * per_xcd + bx / 8;
 }
 
 // (t, c) of a one-channel workgroup over nt x nc; false past the grid.
-__device__ __forceinline__ bool tc_coords(uint32_t per_xcd, int nt, int nc, int &t, int &c) {
-    uint64_t w = xcd_slot(per_xcd);
+// bx: the block index within the grid (a sample's grid in a launch over
+// several samples keeps bx mod 8 = blockIdx.x mod 8, its grid being a
+// multiple of 8).
+__device__ __forceinline__ bool tc_coords(uint32_t per_xcd, int nt, int nc, int &t, int &c,
+                                          uint32_t bx = blockIdx.x) {
+    uint64_t w = xcd_slot(per_xcd, bx);
     const int ci = int(w % kChanGroup);
     w /= kChanGroup;
     t = int(w % uint64_t(nt));
@@ -613,7 +684,7 @@ __device__ __forceinline__ double vis_out(const RimeArgs<T> &A, int t, int b, in
         if (fa[k] > 0) { M.v[k] = {T(0), T(0)}; O.v[k] = {T(0), T(0)}; }
         R.v[k] = {O.v[k].re - M.v[k].re, O.v[k].im - M.v[k].im};
     }
-    jstore(A.model + vi * 4, (A.options & MBX_RESIDUAL_OUTPUT) ? R : M);
+    if (A.model) jstore(A.model + vi * 4, (A.options & MBX_RESIDUAL_OUTPUT) ? R : M);
     T term = T(0);
     if (A.options & MBX_APPLY_WEIGHTS) {
         const T *w = A.wv + vi * 4;
@@ -623,7 +694,7 @@ __device__ __forceinline__ double vis_out(const RimeArgs<T> &A, int t, int b, in
 #pragma unroll
         for (int k = 0; k < 4; ++k) term += R.v[k].re * R.v[k].re + R.v[k].im * R.v[k].im;
     }
-    A.chi2[vi] = term;
+    if (A.chi2) A.chi2[vi] = term;
     return double(term);
 }
 

@@ -1,5 +1,6 @@
-"""Batched evaluation (one persistent launch over many samples) and
-HIP-graph replay.
+"""Batched evaluation (the matrix-core passes over groups of samples, one
+persistent launch over all samples, or the per-sample graph) and HIP-graph
+replay.
 
 Each sample's chi^2 and gradients must equal the numpy / torch float64
 oracle evaluated on that sample alone (float64: 1e-10 relative), and the
@@ -52,17 +53,29 @@ def sample_inputs(shared, s):
     return a
 
 
+MATRIX = {'strategy': 'matrix'}
+MATRIX2 = {'strategy': 'matrix', 'samples_per_group': 2}
+
+
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 @pytest.mark.parametrize('weights', [False, True])
-@pytest.mark.parametrize('strategy', [None, {'strategy': 'per_sample'}])
+@pytest.mark.parametrize('strategy', [None, {'strategy': 'per_sample'}, MATRIX, MATRIX2])
 def test_batched_matches_oracle_per_sample(dtype, weights, strategy):
-    """Both strategies -- the persistent launch (library default shape) and
-    the per-sample matrix-core sequence replayed as a HIP graph -- against
-    the float64 reference gradients of every sample."""
-    from montblanc_amd import batched
+    """Every strategy -- the persistent launch (library default shape), the
+    per-sample matrix-core sequence replayed as a HIP graph, and the
+    matrix-core passes launched over all samples at once or over groups of
+    two (the last group partial) -- against the float64 reference gradients
+    of every sample. The samples include one without point sources and one
+    without gaussian or sersic sources. The matrix strategy is float only:
+    float64 is rejected before any launch."""
+    from montblanc_amd import _lib, batched
     shared, samples = make_batch(dtype=dtype, weights=weights)
     print('batched', np.dtype(dtype).name, weights, strategy, flush=True)
     b = batched.Batch(shared, samples, dtype=dtype)
+    if dtype == np.float64 and strategy in (MATRIX, MATRIX2):
+        with pytest.raises(_lib.MontblancHipError, match='float only'):
+            batched.evaluate(b, gradients=True, use_weight_vector=weights, tune=strategy)
+        return
     X2, g = batched.evaluate(b, gradients=True, use_weight_vector=weights, tune=strategy)
     if strategy is not None:   # a second replay of the captured graph gives the same
         X2 = X2.clone()
@@ -131,6 +144,9 @@ def test_batched_64_antennas_matches_single_model_path():
     shared, samples = make_batch(na=64, ntime=2, nchan=3, counts=counts, seed=33)
     b = batched.Batch(shared, samples, dtype=np.float32)
     X2, g = batched.evaluate(b, gradients=True, tune=None)
+    X2, g = X2.clone(), {n: v.clone() for n, v in g.items()}
+    # the matrix-core passes over all three samples at once
+    X2m, gm = batched.evaluate(b, gradients=True, tune=MATRIX)
     torch.cuda.synchronize()
     for k, s in enumerate(samples):
         a32 = synth.as_dtype(sample_inputs(shared, s), np.float32)
@@ -140,13 +156,23 @@ def test_batched_64_antennas_matches_single_model_path():
         gg = rime.adjoint(arrays, s['counts'], out['ekb_vis'])
         torch.cuda.synchronize()
         assert abs(X2[k].item() - out['X2'].item()) < 1e-6 * out['X2'].item()
+        # the matrix strategy runs the single-model kernels' arithmetic
+        assert abs(X2m[k].item() - out['X2'].item()) < 1e-9 * out['X2'].item()
         o0, o1 = b.src_offset[k], b.src_offset[k + 1]
         up = synth.upcast(a32)
-        names = ('lm', 'stokes', 'alpha')
-        batch = {n: g[n][o0:o1].cpu().numpy() for n in names}
+        names = ('lm', 'stokes', 'alpha', 'gauss_shape')
+        batch = {n: g[n][o0:o1].cpu().numpy() for n in names[:3]}
+        batch['gauss_shape'] = b.split_shapes(g['gauss_shape'], 'gauss')[k].cpu().numpy()
+        matrix = {n: gm[n][o0:o1].cpu().numpy() for n in names[:3]}
+        matrix['gauss_shape'] = b.split_shapes(gm['gauss_shape'], 'gauss')[k].cpu().numpy()
         single = {n: gg[n].cpu().numpy() for n in names}
+        for n in names:
+            assert rel(matrix[n], single[n]) < 1e-5, (k, n)
         print('sample', k, 'batched', check_f32_gradients(up, s['counts'], batch, ref, None,
-                                                          sigma_sqrd=shared['sigma_sqrd'], names=names),
+                                                          sigma_sqrd=shared['sigma_sqrd'], names=names[:3]),
+              flush=True)
+        print('sample', k, 'matrix', check_f32_gradients(up, s['counts'], matrix, ref, None,
+                                                         sigma_sqrd=shared['sigma_sqrd'], names=names),
               flush=True)
         print('sample', k, 'single', check_f32_gradients(up, s['counts'], single, ref,
                                                          out['model_vis'].cpu().numpy(),
@@ -200,3 +226,23 @@ def test_graph_replay_matches_eager():
         torch.cuda.synchronize()
         assert rel(b.X2.cpu(), X2e.cpu()) < 1e-12
         assert rel(b.grads_buffers()['lm'].cpu(), lme.cpu()) < 1e-11
+
+
+def test_matrix_strategy_graph_replay():
+    """The matrix strategy captured in a HIP graph (its clears are fill
+    kernels): every replay, with eager work and input changes between
+    replays, equals the eager result for the current inputs."""
+    from montblanc_amd import batched, graph
+    shared, samples = make_batch(na=11, ntime=2, nchan=5, seed=44)
+    b = batched.Batch(shared, samples, dtype=np.float32)
+    tune = {'strategy': 'matrix', 'samples_per_group': 3}
+    gb = graph.Graphed(lambda: batched.evaluate(b, gradients=True, tune=tune))
+    for rep in range(3):
+        b.arrays['stokes'].mul_(1.0 + 0.05 * rep)
+        X2e, ge = batched.evaluate(b, gradients=True, tune=tune)
+        X2e, lme = X2e.clone(), ge['lm'].clone()
+        torch.cuda.synchronize()
+        gb.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(b.X2.cpu(), X2e.cpu())         # fixed-order reductions
+        assert rel(b.grads_buffers()['lm'].cpu(), lme.cpu()) < 1e-5
