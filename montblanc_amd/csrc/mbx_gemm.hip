@@ -110,29 +110,29 @@ __device__ __forceinline__ double swap_lane_pairs(double v) {
     return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-// SHAPED = false: the point sources [0, npsrc); true: the gaussian and
-// sersic sources [npsrc, nsrc). Separate instances keep each one's
-// registers to what it needs.
-template <typename T, bool SHAPED, bool BATCHED = false>
-__global__ void __launch_bounds__((gemm_threads<T, SHAPED>()))
-k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
+// Index of the pair p <= q in a packed upper triangle of na antennas.
+__host__ __device__ inline int stage_pair(int p, int q, int na) { return p * na - (p * (p - 1)) / 2 + (q - p); }
+
+// Bytes of the staged point sums of every pair p <= q (2 x 2 complex).
+template <typename T> static inline size_t stage_bytes(int na) {
+    return size_t(na) * (na + 1) / 2 * 4 * sizeof(cx<T>);
+}
+
+// The point sources (SHAPED = false) or, in float64, the gaussian and
+// sersic sources (true; float takes k_shaped_mx) of one (t, c), with the
+// pass's wave roles over kGemmThreads threads, from the LDS set-up on.
+// STAGE: the sums of every pair p <= q go to `stage` in LDS (packed upper
+// triangle, stage_pair; the X buffers are dead by then, so the staging
+// area may overlap them) instead of the baselines' running sums in G.dst.
+template <typename T, bool SHAPED, bool STAGE>
+__device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs<T> &G,
+                                            unsigned char *smem_raw, int t, int c, cx<T> *stage) {
     constexpr int kGemmThreads = gemm_threads<T, SHAPED>();
     constexpr int kGemmConsumers = gemm_consumers<SHAPED>();
     constexpr int kGemmMaxTp = gemm_max_tp<SHAPED>();
-    extern __shared__ __align__(32) unsigned char smem_raw[];
-    // in a launch over several samples: this workgroup's sample and slot
-    const SampleAt at = enter_sample_if<BATCHED>(A);
-    if constexpr (BATCHED) {
-        G.dst += uint64_t(at.j) * A.span.vstride;
-        if (G.vin) G.vin += uint64_t(at.j) * A.span.vstride;
-    }
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
-    const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const int tid = threadIdx.x;
-    // XCD order of kChanGroup (mbx_tile.hpp)
-    int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) return;
 
     const int rows = G.rows, st = G.st;
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -296,6 +296,10 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
                 const int row = 16 * tpR[j] + out_row(lane, r);
                 const int p = row >> 1, ip = row & 1;
                 if (p >= NA || q >= NA || p > q) continue;
+                if constexpr (STAGE) {
+                    stage[stage_pair(p, q, NA) * 4 + 2 * ip + kq] = cx<T>{are[j][r], aim[j][r]};
+                    continue;
+                }
                 const int e = G.bmap[p * NA + q];
                 if (e < 0) continue;
                 const bool rev = (e & MBX_BL_REVERSED) != 0;
@@ -359,6 +363,23 @@ k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
             __syncthreads();
         }
     }
+}
+
+template <typename T, bool SHAPED, bool BATCHED = false>
+__global__ void __launch_bounds__((gemm_threads<T, SHAPED>()))
+k_point_gemm(RimeArgs<T> A, GemmArgs<T> G) {
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
+    if constexpr (BATCHED) {
+        G.dst += uint64_t(at.j) * A.span.vstride;
+        if (G.vin) G.vin += uint64_t(at.j) * A.span.vstride;
+    }
+    const int nt_ext = dext(A.d.ntime), nc_ext = dext(A.d.nchan);
+    // XCD order of kChanGroup (mbx_tile.hpp)
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) return;
+    point_phase<T, SHAPED, false>(A, G, smem_raw, t, c, nullptr);
 }
 
 // ---------------------------------------------------------------------
@@ -482,25 +503,17 @@ __device__ __forceinline__ void mx_sources(const XRow<float> *Xp, const XRow<flo
 // epi = 0: the sums go to G.dst (plus G.vin) as in the point pass; epi =
 // 1: the epilogue is applied here (vis_out) and a chi^2 partial per
 // workgroup written to A.partial.
-template <int SETS, bool BATCHED = false>
-__global__ void __launch_bounds__(kMxThreads)
-k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
-    extern __shared__ __align__(32) unsigned char smem_raw[];
-    // in a launch over several samples: this workgroup's sample and slot
-    const SampleAt at = enter_sample_if<BATCHED>(A);
-    if constexpr (BATCHED) {
-        G.dst += uint64_t(at.j) * A.span.vstride;
-        if (G.vin) G.vin += uint64_t(at.j) * A.span.vstride;
-    }
+// FROM_STAGE: the LDS set-up (antenna and channel records) is in place and
+// the point sources' sums of every pair p <= q wait in `stage`
+// (point_phase); the blocks start from them. bx: the workgroup's block
+// index (its chi^2 partial's slot).
+template <int SETS, bool FROM_STAGE>
+__device__ __forceinline__ void shaped_phase(const RimeArgs<float> &A, const GemmArgs<float> &G, int ncw,
+                                             int epi, unsigned char *smem_raw, int t, int c, uint32_t bx,
+                                             const cx<float> *stage) {
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
-    const int nt_ext = dext(d.ntime), nc_ext = dext(d.nchan);
     const int tid = threadIdx.x;
-    int t, c;
-    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) {
-        if (epi && tid == 0) A.partial[at.bx] = 0.0;   // padding workgroup
-        return;
-    }
 
     const int st = G.st;
     double *Luvw = reinterpret_cast<double *>(smem_raw);
@@ -538,16 +551,20 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
         if (ntiles > 0) records(0);
         if (ntiles > 1) records(1);
     }
-
-    tile_setup<float>(A, Luvw, Lpa, Lch, t, c, tid, kMxThreads);
-    stage_antennas<float>(A, Lant, t, c, tid, kMxThreads);
     // pad antennas of both buffers: finite values for the reads past the end
-    for (int k = tid; k < 2 * kMxMaxSets; k += kMxThreads) {
-        const int b = k / kMxMaxSets, a = st * NA + k % kMxMaxSets;
-        for (int r = 0; r < 4; ++r) Xbase[b * xstride + 4 * a + r] = XRow<float>{{0.f, 0.f, 0.f, 0.f}};
-        XYbase[b * bufa + a] = float2{0.f, 0.f};
+    auto pad_clear = [&]() {
+        for (int k = tid; k < 2 * kMxMaxSets; k += kMxThreads) {
+            const int b = k / kMxMaxSets, a = st * NA + k % kMxMaxSets;
+            for (int r = 0; r < 4; ++r) Xbase[b * xstride + 4 * a + r] = XRow<float>{{0.f, 0.f, 0.f, 0.f}};
+            XYbase[b * bufa + a] = float2{0.f, 0.f};
+        }
+    };
+    if constexpr (!FROM_STAGE) {
+        tile_setup<float>(A, Luvw, Lpa, Lch, t, c, tid, kMxThreads);
+        stage_antennas<float>(A, Lant, t, c, tid, kMxThreads);
+        pad_clear();
+        __syncthreads();
     }
-    __syncthreads();
 
     if (wave < ncw) {
         const int lane = tid & 63, blk = lane >> 2, i = lane & 3;
@@ -568,6 +585,24 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
         f32x4 D[SETS][2];
 #pragma unroll
         for (int g = 0; g < SETS; ++g) D[g][0] = D[g][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (FROM_STAGE) {
+            // the point sums of this lane's pairs (p_g, q), p_g <= q
+#pragma unroll
+            for (int g = 0; g < SETS; ++g) {
+                const int p = p0 + g;
+                if (!qok || p >= span || p > q) continue;
+                const cx<float> *v = stage + stage_pair(p, q, NA) * 4;
+#pragma unroll
+                for (int ip = 0; ip < 2; ++ip)
+#pragma unroll
+                    for (int kq = 0; kq < 2; ++kq) {
+                        D[g][kq][ip] = v[2 * ip + kq].re;
+                        D[g][kq][2 + ip] = v[2 * ip + kq].im;
+                    }
+            }
+            __syncthreads();                  // staged sums read: the buffers are free
+            pad_clear();
+        }
         __syncthreads();                      // tile 0 produced
         for (int it = 0; it < ntiles; ++it) {
             const int buf = it & 1;
@@ -648,6 +683,10 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
         // records of tile it + 2 go to buffer it & 1 while tile it + 1 is
         // produced (tile it's records were last read before the barrier
         // that ended the previous step)
+        if constexpr (FROM_STAGE) {
+            __syncthreads();                  // staged sums read: the buffers are free
+            pad_clear();
+        }
         if (ntiles > 0) produce(0);
         __syncthreads();
         for (int it = 0; it < ntiles; ++it) {
@@ -656,7 +695,53 @@ k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
             __syncthreads();
         }
     }
-    if (epi) wg_partial(term_sum, Lpart, A.partial + at.bx, tid, kMxThreads);
+    if (epi) wg_partial(term_sum, Lpart, A.partial + bx, tid, kMxThreads);
+}
+
+template <int SETS, bool BATCHED = false>
+__global__ void __launch_bounds__(kMxThreads)
+k_shaped_mx(RimeArgs<float> A, GemmArgs<float> G, int ncw, int epi) {
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
+    if constexpr (BATCHED) {
+        G.dst += uint64_t(at.j) * A.span.vstride;
+        if (G.vin) G.vin += uint64_t(at.j) * A.span.vstride;
+    }
+    const int nt_ext = dext(A.d.ntime), nc_ext = dext(A.d.nchan);
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) {
+        if (epi && threadIdx.x == 0) A.partial[at.bx] = 0.0;   // padding workgroup
+        return;
+    }
+    shaped_phase<SETS, false>(A, G, ncw, epi, smem_raw, t, c, at.bx, nullptr);
+}
+
+// The whole float forward of one (t, c) in one workgroup: the point
+// sources with the point pass's wave roles (Gp), their sums of every pair
+// p <= q staged in LDS over the dead X buffers, then the gaussian and
+// sersic sources on 4x4 blocks starting from those sums (Gs, ncw consumer
+// waves), and the epilogue. No running sums pass through device memory.
+static_assert(gemm_threads<float, false>() == kMxThreads, "both phases use the same workgroup");
+template <int SETS, bool BATCHED = false>
+__global__ void __launch_bounds__(kMxThreads)
+k_forward_mx(RimeArgs<float> A, GemmArgs<float> Gp, GemmArgs<float> Gs, int ncw) {
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    // in a launch over several samples: this workgroup's sample and slot
+    const SampleAt at = enter_sample_if<BATCHED>(A);
+    if constexpr (BATCHED) {
+        if (Gs.vin) Gs.vin += uint64_t(at.j) * A.span.vstride;
+    }
+    const int nt_ext = dext(A.d.ntime), nc_ext = dext(A.d.nchan);
+    int t, c;
+    if (!tc_coords(tc_per_xcd(nt_ext, nc_ext), nt_ext, nc_ext, t, c, at.bx)) {
+        if (threadIdx.x == 0) A.partial[at.bx] = 0.0;   // padding workgroup
+        return;
+    }
+    cx<float> *stage = reinterpret_cast<cx<float> *>(smem_raw + gemm_x_offset(A.d.na.local_size));
+    point_phase<float, false, true>(A, Gp, smem_raw, t, c, stage);
+    __syncthreads();                           // staged sums complete
+    shaped_phase<SETS, true>(A, Gs, ncw, 1, smem_raw, t, c, at.bx, stage);
 }
 
 static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_t nwg, int epi,
@@ -674,6 +759,56 @@ static int launch_shaped_mx(const RimeArgs<float> &A, GemmArgs<float> G, uint64_
     default: hipLaunchKernelGGL((k_shaped_mx<4>), grid, block, p.lds, stream, A, G, p.waves, epi); break;
     }
     return hip_status(hipGetLastError(), "shaped matrix pass launch");
+}
+
+// The fused float forward (k_forward_mx): the point phase's source tile,
+// the shaped phase's plan and the LDS of both; false where the staged sums
+// would reach the shaped phase's source records (written while the sums
+// are read) or the LDS does not hold both phases.
+struct FwdMxPlan {
+    MxPlan mx;
+    int stp;
+    size_t lds;
+};
+
+static bool forward_mx_plan(int na, size_t budget, FwdMxPlan &f) {
+    if (!mx_plan(na, budget, f.mx)) return false;
+    f.stp = gemm_st<float, false>(na, budget);
+    const size_t xy_bytes = 2 * mx_buf_ants(na, f.mx.st) * (4 * sizeof(XRow<float>) + sizeof(float2));
+    if (stage_bytes<float>(na) > xy_bytes) return false;
+    f.lds = std::max(gemm_lds_bytes<float, false>(na, f.stp), f.mx.lds);
+    return f.lds <= budget;
+}
+
+// k_forward_mx over `grid` workgroups; vin: the running sums of earlier
+// source batches, or NULL.
+template <bool BATCHED>
+static int launch_forward_mx(const RimeArgs<float> &A, const int32_t *bmap, const cx<float> *vin,
+                             const FwdMxPlan &f, uint64_t grid, hipStream_t stream) {
+    const int na = A.d.na.local_size;
+    GemmArgs<float> Gp;
+    Gp.bmap = bmap; Gp.dst = nullptr; Gp.vin = nullptr;
+    Gp.rows = gemm_rows(na);
+    const int nR = Gp.rows / 16;
+    Gp.ntp = nR * (nR + 1) / 2;
+    Gp.st = f.stp;
+    constexpr int cw = gemm_consumers<false>() / 64;
+    Gp.tpw = (Gp.ntp + cw - 1) / cw;
+    if (Gp.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "fused forward: %d antennas", na);
+    GemmArgs<float> Gs = Gp;
+    Gs.vin = vin;
+    Gs.st = f.mx.st;
+    RimeArgs<float> Ag = A;
+    Ag.ct = 1;
+    const dim3 g{unsigned(grid)}, block{unsigned(kMxThreads)};
+    const int w = f.mx.waves;
+    switch (f.mx.sets) {
+    case 1: hipLaunchKernelGGL((k_forward_mx<1, BATCHED>), g, block, f.lds, stream, Ag, Gp, Gs, w); break;
+    case 2: hipLaunchKernelGGL((k_forward_mx<2, BATCHED>), g, block, f.lds, stream, Ag, Gp, Gs, w); break;
+    case 3: hipLaunchKernelGGL((k_forward_mx<3, BATCHED>), g, block, f.lds, stream, Ag, Gp, Gs, w); break;
+    default: hipLaunchKernelGGL((k_forward_mx<4, BATCHED>), g, block, f.lds, stream, Ag, Gp, Gs, w); break;
+    }
+    return hip_status(hipGetLastError(), "fused forward launch");
 }
 
 template <typename T>
@@ -717,6 +852,15 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
     RimeArgs<T> Ag = A;
     Ag.ct = 1;
     const int np = dext(d.npsrc), ns = dext(d.nsrc);
+    if constexpr (sizeof(T) == 4) {
+        // float with the epilogue: both passes in one kernel where it fits
+        FwdMxPlan f;
+        if (shaped && epilogue_wgs && np > 0 && forward_mx_plan(na, budget, f)) {
+            const int r = launch_forward_mx<false>(Ag, bmap, vin, f, nwg, stream);
+            if (!r) *epilogue_wgs = nwg;
+            return r;
+        }
+    }
     if (np > 0) {
         G.st = gemm_st<T, false>(na, budget);
         const size_t lds = gemm_lds_bytes<T, false>(na, G.st);
@@ -769,6 +913,9 @@ int launch_batched_mx(const RimeArgs<float> &A, const int32_t *bmap, cx<float> *
     static size_t budget = gemm_lds_budget();
     RimeArgs<float> Ag = A;
     Ag.ct = 1;
+    FwdMxPlan f;
+    if (any_point && forward_mx_plan(na, budget, f))   // one launch: both passes in one kernel
+        return launch_forward_mx<true>(Ag, bmap, nullptr, f, nwg * nrep, stream);
     if (any_point) {
         G.st = gemm_st<float, false>(na, budget);
         const size_t lds = gemm_lds_bytes<float, false>(na, G.st);

@@ -160,11 +160,12 @@ template <typename T> bool shaped_gemm_pays(const mbx_dims &d);
 // [0, npsrc extent) of J_p J_q^H -- and with `shaped` over the gaussian
 // and sersic sources too, f_pq J_p J_q^H -- for every baseline of the map.
 //
-// With `epilogue_wgs` set, a float shaped pass (k_shaped_mx) also applies
-// the epilogue (gains, flags, model or residual, chi^2 terms; A's output
-// arrays) and writes one chi^2 partial per workgroup to A.partial; the
-// number of workgroups (gemm_workgroups) is stored there, 0 when no pass
-// applied the epilogue.
+// With `epilogue_wgs` set, a float shaped pass (k_shaped_mx, or with point
+// sources both passes in one kernel, k_forward_mx, whose point sums stay
+// in LDS and never reach dst) also applies the epilogue (gains, flags,
+// model or residual, chi^2 terms; A's output arrays) and writes one chi^2
+// partial per workgroup to A.partial; the number of workgroups
+// (gemm_workgroups) is stored there, 0 when no pass applied the epilogue.
 template <typename T>
 int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, const cx<T> *vin,
                       bool shaped, hipStream_t stream, uint64_t *epilogue_wgs = nullptr);
@@ -173,13 +174,13 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
 // to the XCD order).
 uint64_t gemm_workgroups(const mbx_dims &d);
 
-// Forward of nrep samples in one launch per pass (float; A.span set with
-// wgs = gemm_workgroups(A.d)): the point sources of every sample into its
-// slot of buf (nrep slots of A.span.vstride complex values), then the
-// gaussian and sersic sources on top, with the epilogue: chi^2 partials
-// per sample, A's model / chi^2 term arrays where set, and the pre-gain
-// sums where A.ekb is set (to buf itself, slot by slot). any_point: some
-// sample has point sources.
+// Forward of nrep samples (float; A.span set with wgs =
+// gemm_workgroups(A.d)): one launch of the fused kernel where it fits
+// (else the point sources of every sample into its slot of buf -- nrep
+// slots of A.span.vstride complex values -- then the gaussian and sersic
+// sources on top), with the epilogue: chi^2 partials per slot, A's model
+// / chi^2 term arrays where set, and the pre-gain sums where A.ekb is set
+// (to buf, slot by slot). any_point: some sample has point sources.
 int launch_batched_mx(const RimeArgs<float> &A, const int32_t *bmap, cx<float> *buf, bool any_point,
                       unsigned nrep, hipStream_t stream);
 bool batched_mx_supported(const mbx_dims &d);
