@@ -437,8 +437,8 @@ def main():
         'roofline': {'bound': 'mfma', 'achieved': r['tflops'], 'peak': peak,
                      'unit': 'TFLOP/s', 'frac': r['tflops'] / peak,
                      'traffic': measured_traffic(args.workload, primary),
-                     'kernel': 'mbx_forward_%s (k_beam_planes, k_point_gemm, k_shaped_mx with '
-                               'the epilogue, k_chi2_final)' % primary,
+                     'kernel': 'mbx_forward_%s (k_beam_planes; k_forward_mx: point and gaussian/sersic '
+                               'passes and the epilogue in one kernel; k_chi2_final)' % primary,
                      'flops_per_launch': r['flops'],
                      'peak_note': 'compute-bound: point sources on the matrix cores (16x16x4 '
                                   'products), gaussian and sersic sources on the matrix cores '
@@ -447,11 +447,10 @@ def main():
                                   'cores, the same figure as the packed-FP32 vector peak; this '
                                   'build compiles packed FP32 out, leaving a 78.6 TFLOP/s vector '
                                   'rate)',
-                     'limiters': 'k_point_gemm: texture-data path 63%, matrix cores 53%, vector '
-                                 'units 45%; k_shaped_mx: matrix cores 32%, vector issue 68% of '
-                                 'busy cycles, LDS bank conflicts 13% (rocprofv3 PMC, '
-                                 'forward_64ant f32; k_shaped_mx counted before its scaled B '
-                                 'operands were scheduled ahead of the products, 12.2 -> 11.3 ms)'},
+                     'limiters': 'k_forward_mx: matrix cores busy 46% of SIMD cycles, other vector '
+                                 'instructions 43% (rocprofv3 PMC, forward_64ant f32): 89% together, '
+                                 'matrix and vector issue taking turns on each SIMD; texture path '
+                                 '37% (profiles/forward_64ant_f32_counters.md)'},
         'timing_note': 'value and ms_per_step: wall clock over the timed steps with the device '
                        'synchronised on both sides; ms_event_median / terms_per_s_median: the '
                        'median of per-step HIP events (the reference protocol: warm-up, then the '
