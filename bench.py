@@ -26,7 +26,7 @@ times, 256 channels, 12000 sources: forward and forward + gradient, with
 the error against the float64 oracle on a slice), the attainable library
 GEMM rates and the CPU baselines.
 
-Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
 
 import argparse
@@ -43,7 +43,7 @@ sys.path.insert(0, ROOT)
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument('--gpus', type=int, default=1)
-    p.add_argument('--steps', type=int, default=10)
+    p.add_argument('--steps', type=int, default=20)
     p.add_argument('--warmup', type=int, default=3)
     p.add_argument('--workload', default='forward_64ant')
     p.add_argument('--dtypes', default='f32,f64')
