@@ -114,6 +114,34 @@ def test_batched_matches_oracle_per_sample(dtype, weights, strategy):
                                          sigma_sqrd=1.3, names=tuple(got)))
 
 
+@pytest.mark.parametrize('strategy,dtype', [(None, np.float64), ({'strategy': 'per_sample'}, np.float64),
+                                            (MATRIX, np.float32)])
+def test_batched_sample_without_sources(strategy, dtype):
+    """A sample with no sources at all: its chi^2 is that of a zero model
+    (the oracle's), and the other samples' chi^2 and gradients are those of
+    the same batch without it."""
+    from montblanc_amd import batched
+    counts = [(3, 1, 0), (0, 0, 0), (2, 2, 1)]
+    shared, samples = make_batch(na=9, ntime=2, nchan=3, counts=counts, seed=61)
+    print('empty sample', strategy, np.dtype(dtype).name, flush=True)
+    b = batched.Batch(shared, samples, dtype=dtype)
+    X2, g = batched.evaluate(b, gradients=True, tune=strategy)
+    torch.cuda.synchronize()
+    X2 = X2.cpu().numpy()
+    g = {k: v.cpu().numpy() for k, v in g.items()}
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    ref = rime_np.rime(synth.upcast(synth.as_dtype(sample_inputs(shared, samples[1]), dtype)), 0, 0, 0,
+                       sigma_sqrd=shared['sigma_sqrd'])
+    assert abs(X2[1] - ref['X2']) <= tol * ref['X2']
+    kept = [samples[0], samples[2]]
+    b2 = batched.Batch(shared, kept, dtype=dtype)
+    X2b, gb = batched.evaluate(b2, gradients=True, tune=strategy)
+    torch.cuda.synchronize()
+    assert rel(X2[[0, 2]], X2b.cpu().numpy()) < 1e-12
+    for name in ('lm', 'stokes', 'alpha', 'gauss_shape', 'sersic_shape'):
+        assert rel(g[name], gb[name].cpu().numpy()) < (1e-11 if dtype == np.float64 else 1e-5), name
+
+
 @pytest.mark.parametrize('tune', [None, {'workgroups': 1}, {'workgroups': 3, 'channels_per_item': 2},
                                   {'sources_per_tile': 2, 'workgroups': 17}])
 def test_batched_launch_shapes_agree(tune):
