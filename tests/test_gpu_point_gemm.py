@@ -78,10 +78,12 @@ def test_shaped_matrix_pass_many_sources(dtype, na):
 
 @pytest.mark.parametrize('residual,use_w', [(False, False), (True, False), (False, True)])
 def test_shaped_matrix_pass_applies_epilogue(residual, use_w):
-    """In float from 40 antennas the shaped matrix pass applies the
-    epilogue itself: gains, flags, residual mode, weights, chi^2 terms and
-    their sum, with baselines in either antenna order (the baseline's own
-    order picks the gains) and auto-correlations."""
+    """In float from 40 antennas the matrix-core kernel applies the
+    epilogue itself (here k_forward_mx: point sources, their sums staged in
+    LDS, then the gaussian and sersic sources): gains, flags, residual
+    mode, weights, chi^2 terms and their sum, with baselines in either
+    antenna order (the baseline's own order picks the gains) and
+    auto-correlations."""
     from test_gpu_forward import check_outputs, run_forward
     counts = (5, 6, 4)
     a = synth.make_inputs(40, 2, 3, *counts, beam=(9, 8, 6), seed=77, flag_fraction=0.3,
@@ -103,7 +105,8 @@ def test_shaped_matrix_pass_applies_epilogue(residual, use_w):
 @pytest.mark.parametrize('dtype,na', [(np.float32, 44), (np.float64, 48)])
 def test_shaped_matrix_pass_without_point_sources(dtype, na):
     """No point sources: the point pass is skipped and the shaped matrix
-    pass starts from no running sum (and in float applies the epilogue)."""
+    pass (k_shaped_mx in float, on its own) starts from no running sum (and
+    in float applies the epilogue)."""
     counts = (0, 6, 4)
     a = synth.make_inputs(na, 2, 3, *counts, beam=(9, 8, 6), seed=na + 1, flag_fraction=0.2)
     check(a, counts, dtype)
