@@ -244,12 +244,25 @@ def alloc_outputs(arrays, keep_ekb=False):
     return out
 
 
-# Default kernel form per dtype: (pb, qb, producer_waves) -- pair blocks of
-# pb x qb antenna pairs per consumer thread; producer_waves 0 = the
-# double-buffered kernel with an automatic producer count, < 0 = the
-# single-role pair-block kernel; 'generic' = per-baseline kernel (measured
-# fastest in double, where the pair-block accumulators spill).
-DEFAULT_PAIR_BLOCK = {'f32': (2, 2, 0), 'f64': 'generic'}
+def default_pair_block(sfx, na):
+    """Default form of the vector-unit forward kernel: (pb, qb,
+    producer_waves) -- pair blocks of pb x qb antenna pairs per consumer
+    thread; producer_waves 0 = the double-buffered producer / consumer
+    kernel with an automatic producer count, < 0 = the single-role
+    pair-block kernel; 'generic' = the per-baseline kernel.
+
+    Measured on MI355X (forward, median of 5): up to 64 antennas, where the
+    point sources run on the matrix cores and this kernel sums the other
+    sources and applies the epilogue, (1, 1, 0) in both dtypes (27
+    antennas, 200 times, 128 channels, 300 point + 200 gaussian: float
+    10.37 ms against 11.65 for (2, 2, 0) and 10.91 generic; float64 23.17
+    against 25.23 generic; also ahead at 14 and 16 antennas). Above 64
+    antennas every source runs here: float (2, 2, 0), float64 (2, 2, -1)
+    (100 antennas, 320 sources: 1.79 and 5.91 ms against 5.50 and 8.42
+    generic, 5.73 and 15.48 for (1, 1, 0))."""
+    if na <= 64:
+        return (1, 1, 0)
+    return (2, 2, 0) if sfx == 'f32' else (2, 2, -1)
 
 
 def build_plan(antenna1, antenna2, na, device, form, point_gemm=True):
@@ -289,9 +302,9 @@ def build_plan(antenna1, antenna2, na, device, form, point_gemm=True):
     return (_lib.mbx_pair_plan(blocks, nblocks, pb, qb, pw, bmap, 1 if bmap else 0), keep)
 
 
-def _plan_key(sfx, pair_block, point_gemm):
+def _plan_key(sfx, pair_block, point_gemm, na):
     if pair_block is None:
-        pair_block = DEFAULT_PAIR_BLOCK[sfx]
+        pair_block = default_pair_block(sfx, na)
     return ('_pair_plan', pair_block if pair_block == 'generic' else tuple(pair_block),
             bool(point_gemm)), pair_block
 
@@ -300,7 +313,7 @@ def plan_entry(antenna1, antenna2, na, device, sfx, pair_block=None, point_gemm=
     """The '_pair_plan' cache entry of a forward call, built from HOST
     antenna arrays (no device-to-host copy): callers that hold the host
     layout (the chunked solver) put it in the arrays dict beforehand."""
-    key, form = _plan_key(sfx, pair_block, point_gemm)
+    key, form = _plan_key(sfx, pair_block, point_gemm, na)
     return (key, build_plan(np.asarray(antenna1), np.asarray(antenna2), na, device, form, point_gemm))
 
 
@@ -309,9 +322,9 @@ def pair_plan(arrays, pair_block=None, point_gemm=True):
     the dict under '_pair_plan'); None selects the generic kernel.
 
     pair_block: (pb, qb[, producer_waves]), 'generic', or None for the
-    dtype default.
+    default of the dtype and antenna count (default_pair_block).
     """
-    key, form = _plan_key(suffix(arrays), pair_block, point_gemm)
+    key, form = _plan_key(suffix(arrays), pair_block, point_gemm, arrays['uvw'].shape[1])
     cached = arrays.get('_pair_plan')
     if cached is not None and cached[0] == key:
         return cached[1]

@@ -360,8 +360,8 @@ class RimeSolver(object):
         """Device launch plan of the current antenna layout (rebuilt after
         antenna1/antenna2 transfers); None for the generic kernel."""
         if self._pair_plan is None:
-            from .rime import DEFAULT_PAIR_BLOCK, build_plan
-            form = DEFAULT_PAIR_BLOCK['f32' if self.is_float() else 'f64']
+            from .rime import build_plan, default_pair_block
+            form = default_pair_block('f32' if self.is_float() else 'f64', self.dim_local_size('na'))
             for i, name in enumerate(('antenna1', 'antenna2')):
                 if self._host_antenna[i] is None:
                     self._host_antenna[i] = self.retrieve(name)
