@@ -133,9 +133,11 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
     const double gl0 = floor(vl), gm0 = floor(vm);
     const T ld = T(vl - gl0), md = T(vm - gm0);
     const uint32_t MH = d.beam_mh.local_size;
-    const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
-    const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const BeamCell<T> *plane = A.beam + uint64_t(c) * d.beam_lw.local_size * MH;
+    // tap byte offsets in the channel's plane, as in beam_jones_at
+    const uint32_t cell = uint32_t(sizeof(BeamCell<T>));
+    const uint32_t o00 = uint32_t(__umul24(uint32_t(__umul24(uint32_t(gl0), MH)) + uint32_t(gm0), cell));
+    const uint32_t dl = gl0 < LW1 ? uint32_t(__umul24(MH, cell)) : 0u, dm = gm0 < MH1 ? cell : 0u;
+    const char *plane = reinterpret_cast<const char *>(A.beam + uint64_t(c) * d.beam_lw.local_size * MH);
     cx<T> S[4], dSl[4], dSm[4];
     T Am[4], dAl[4], dAm[4];
 #pragma unroll
@@ -148,12 +150,11 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
 #pragma unroll 1
     for (int tap = 0; tap < 4; ++tap) {
         const bool hl = tap & 1, hm = (tap >> 1) & 1;
-        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0;
         const T wl = hl ? ld : T(1) - ld, wm = hm ? md : T(1) - md;
         const T w = wl * wm;
         const T wdl = hl ? wm : -wm;
         const T wdm = hm ? wl : -wl;
-        const BeamCell<T> e = plane[il * MH + im];
+        const BeamCell<T> e = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + (hl ? dl : 0u) + (hm ? dm : 0u));
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
             const T vre = e.v[3 * x], vim = e.v[3 * x + 1], vamp = e.v[3 * x + 2];

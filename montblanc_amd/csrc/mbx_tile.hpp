@@ -446,9 +446,13 @@ __device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, 
     double gm0 = floor(vm);
     const T ld = T(vl - gl0), md = T(vm - gm0);
     const uint32_t MH = d.beam_mh.local_size;
-    const uint32_t il0 = uint32_t(gl0), il1 = uint32_t(fmin(gl0 + 1.0, LW1));
-    const uint32_t im0 = uint32_t(gm0), im1 = uint32_t(fmin(gm0 + 1.0, MH1));
-    const BeamCell<T> *plane = A.beam + uint64_t(c) * d.beam_lw.local_size * MH;
+    // byte offsets of the four taps in the channel's plane (a plane is a
+    // few MB at most: 32-bit, 24-bit multiplies); the + 1 neighbours
+    // clamp to the last row / column (g1 = min(g0 + 1, N - 1))
+    const uint32_t cell = uint32_t(sizeof(BeamCell<T>));
+    const uint32_t o00 = uint32_t(__umul24(uint32_t(__umul24(uint32_t(gl0), MH)) + uint32_t(gm0), cell));
+    const uint32_t dl = gl0 < LW1 ? uint32_t(__umul24(MH, cell)) : 0u, dm = gm0 < MH1 ? cell : 0u;
+    const char *plane = reinterpret_cast<const char *>(A.beam + uint64_t(c) * d.beam_lw.local_size * MH);
     cx<T> acc[4];
     T amp[4];
 #pragma unroll
@@ -460,9 +464,8 @@ __device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, 
     for (int u = 0; u < UT; ++u) {
         const int tap = tap0 + u;
         const bool hl = tap & 1, hm = (tap >> 1) & 1;
-        const uint32_t il = hl ? il1 : il0, im = hm ? im1 : im0;
         const T w = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md);
-        const BeamCell<T> e = plane[il * MH + im];
+        const BeamCell<T> e = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + (hl ? dl : 0u) + (hm ? dm : 0u));
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             acc[p].re = fma(w, e.v[3 * p], acc[p].re);
