@@ -3,7 +3,10 @@
 // k_point_gemm<T, true> (the 16x16 products scaled per element by the
 // shape factor) and in float in k_shaped_mx (4x4-block products with the
 // factor on the B operand; it also applies the epilogue), both described
-// at their kernels.
+// at their kernels. In float k_forward_mx runs the point pass and the
+// shaped pass one after the other in one workgroup, the point sums handed
+// over in LDS. Every kernel has a BATCHED instance that covers several
+// samples' grids in one launch (SampleSpan, mbx_tile.hpp).
 //
 // For one time t and channel c the point sources' part of every
 // baseline's coherency sum (shape factor 1; CPUSolver.py:562-634,
@@ -14,11 +17,12 @@
 // In real arithmetic, with A = [Re X | Im X] and A' = [Im X | -Re X]
 // (2na x 4S each):  Re V = A A^T,  Im V = A' A^T.
 //
-// Kernel k_point_gemm: one workgroup per (t, c), eight waves.
-//   producer waves (4-7): per-antenna Jones of the next tile of `st`
+// Kernel k_point_gemm: one workgroup per (t, c); four consumer waves and
+// twelve (float) or four (float64) producer waves.
+//   producer waves: per-antenna Jones of the next tile of `st`
 //     sources into one of two LDS buffers, laid out X[s][row] with
 //     row = 2p + i and the four reals (Re J_i0, Re J_i1, Im J_i0, Im J_i1);
-//   consumer waves (0-3): own 16x16 output tiles (R, C), R <= C, of both
+//   consumer waves (waves 0-3): own 16x16 output tiles (R, C), R <= C, of both
 //     Re V and Im V ("tile pairs") in MFMA accumulators and step through
 //     the other buffer four sources at a time with
 //     v_mfma_{f32,f64}_16x16x4: lane group g = lane / 16 supplies source
@@ -28,8 +32,8 @@
 // Only the upper triangle of tiles is computed (V_qp = V_pq^H). The
 // epilogue scatters each (p <= q) element to its baseline through the
 // host-built pair -> baseline map (reversed baselines get the conjugate
-// transpose) into the running-sum buffer the fused kernel then finishes
-// (other source types, gains, flags, chi^2).
+// transpose) into the running-sum buffer that a later pass finishes (other
+// source types, gains, flags, chi^2) -- or, inside k_forward_mx, into LDS.
 
 #include <algorithm>
 
