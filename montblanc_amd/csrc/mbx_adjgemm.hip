@@ -21,7 +21,9 @@
 // 2 Re <J_p, Vb J_q> per baseline).
 // Every pass then runs the per-(source, antenna) backward of J = E K
 // sqrt(B) and reduces each source's gradient in LDS before one set of
-// float64 atomics per source and window of source tiles.
+// float64 atomics per source and window of source tiles. BATCHED
+// instances cover several samples' grids in one launch (SampleSpan,
+// mbx_tile.hpp; enter_sample for AdjArgs, mbx_backward.hpp).
 
 #include "mbx_backward.hpp"
 #include "mbx_gemm.hpp"

@@ -1,9 +1,16 @@
 // Batched RIME: many sky-model samples (each with its own, unevenly sized
 // source list) evaluated against one shared set of visibility-space inputs
-// (uvw, beam, gains, observed visibilities, flags, weights) in ONE
-// persistent launch. No reference counterpart.
+// (uvw, beam, gains, observed visibilities, flags, weights). No reference
+// counterpart. Two strategies (mbx_batch_tuning.strategy):
 //
-// Work items are (sample, time, channel tile); the host orders samples by
+// 1: the matrix-core kernels of the single-model path (BATCHED instances
+// of k_forward_mx, k_adj_vis, k_adj_point, k_adj_gemm) each launched once
+// over a group of samples, every sample a grid of (time, channel)
+// workgroups (SampleSpan, mbx_tile.hpp); per-slot chi^2 from each slot's
+// partials (k_chi2_slots). Host loop: batched_mx below.
+//
+// 0: ONE persistent launch on the vector units, k_batched below. Work
+// items are (sample, time, channel tile); the host orders samples by
 // decreasing source count so the longest items start first, and the
 // workgroups pull items from an atomic counter until the queue is empty.
 // Per item a workgroup
