@@ -524,6 +524,8 @@ static int batched(const mbx_dims *d, const mbx_inputs *in, const mbx_batch *bt,
     const bool mx = tune && tune->strategy == 1;
     BatPlan p;
     if (!mx && (r = plan_batched<T>(*d, tune, p))) return r;
+    // shapes the matrix strategy rejects: its own checks report them
+    // before the workspace is sized (its size function returns 0 for them)
     if (mx && (sizeof(T) == 8 || !bmap || tune->samples_per_group <= 0))
         return batched_mx<T>(d, in, bt, X2, gr, grads, bmap, b, k, tune, options, ws, ws_bytes, stream);
     size_t need = batched_ws_bytes<T>(*d, tune);
