@@ -23,6 +23,7 @@ What changes for one MI355X:
 
 import logging
 import math
+import os
 import re
 from collections import OrderedDict
 
@@ -165,6 +166,20 @@ STATIC_ARRAYS = tuple(n for n, shape, _dt, _d, _t, scratch in schema.ARRAYS
 INT_ARRAYS = ('antenna1', 'antenna2')
 
 
+def pin_limit():
+    """Host bytes the chunked solver may keep page-locked for its full-size
+    arrays: a quarter of the physical memory, at most 16 GiB; 0 without a
+    device (page-locked memory belongs to the device runtime)."""
+    import torch
+    if not torch.cuda.is_available():
+        return 0
+    try:
+        phys = os.sysconf('SC_PHYS_PAGES') * os.sysconf('SC_PAGE_SIZE')
+    except (ValueError, OSError, AttributeError):
+        phys = 0
+    return int(min(16 << 30, phys // 4))
+
+
 def _device_ready(name, v):
     """A host array in the dtype the device path takes (as rime.to_device)."""
     if name in INT_ARRAYS:
@@ -187,11 +202,15 @@ class ChunkedRimeSolver(object):
     (CompositeRimeSolver.py:793-1012, 1027-1215), on one device.
 
     The loop is pipelined: chunk k runs on the current stream while a copy
-    stream moves chunk k + 1's inputs in through pinned host buffers and
-    chunk k - 1's outputs out, so the host-device copies overlap the
-    kernels. Two chunks are then resident at once: the planner is given
-    half of the device budget. Arrays without time or channel axes stay on
-    the device between chunks and solves (STATIC_ARRAYS).
+    stream moves chunk k + 1's inputs in and chunk k - 1's outputs out, so
+    the host-device copies overlap the kernels. Two chunks are then
+    resident at once: the planner is given half of the device budget.
+    Arrays without time or channel axes stay on the device between chunks
+    and solves (STATIC_ARRAYS). Where the full-size host arrays fit
+    pin_limit() they are page-locked, and a chunk's slices that are
+    contiguous in them (time-major arrays, chunks over time only) go to and
+    from the device directly; other slices pass through page-locked staging
+    buffers.
     """
 
     def __init__(self, slvr_cfg):
@@ -207,7 +226,7 @@ class ChunkedRimeSolver(object):
         self.device = resolve_device(slvr_cfg.get(Options.CONTEXT))
         # a host-side twin holds dims, properties and the full arrays
         cfg = dict(slvr_cfg)
-        self._host = _HostArrays(cfg, self.ft, self.ct)
+        self._host = _HostArrays(cfg, self.ft, self.ct, pin_limit())
         self.X2 = self.ft(0.0)
         self.last_plan = None
         self.last_solve = None
@@ -266,10 +285,8 @@ class ChunkedRimeSolver(object):
         nt, nc = h.dims['ntime'], h.dims['nchan']
         counts = (h.dims['npsrc'], h.dims['ngsrc'], h.dims['nssrc'])
         jobs = [(t0, t1, c0, c1) for t0, t1 in chunks(nt, T['ntime']) for c0, c1 in chunks(nc, T['nchan'])]
-        model = h.arrays['model_vis']
+        model = h.arrays['model_vis']      # every chunk overwrites (or adds to) its slice
         chi = h.arrays['chi_sqrd_result']
-        if not self._sum:
-            model[...] = 0
         dev = self.device
         comp = torch.cuda.current_stream(dev)
         if self._copy is None:
@@ -305,20 +322,29 @@ class ChunkedRimeSolver(object):
                         arrays[k] = v
                         continue
                     src = torch.from_numpy(_device_ready(k, v))
-                    pin = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
-                    pin.copy_(src)
                     d = torch.empty(src.shape, dtype=src.dtype, device=dev)
-                    d.copy_(pin, non_blocking=True)
+                    if h.pinned and src.is_pinned():
+                        d.copy_(src, non_blocking=True)      # a view of the page-locked array
+                    else:
+                        pin = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
+                        pin.copy_(src)
+                        d.copy_(pin, non_blocking=True)
                     d.record_stream(comp)
                     arrays[k] = d
                 ready = torch.cuda.Event()
                 ready.record(copy)
             return arrays, ready
 
+        # outputs straight into the page-locked host arrays where the chunk
+        # is contiguous in them (and the model is overwritten, not summed)
+        direct = h.pinned and not self._sum and T['nchan'] == nc
+
         def drain(p):
             # a finished chunk's outputs into the host arrays
             (t0, t1, c0, c1), mv, cs, done = p
             done.synchronize()
+            if direct:
+                return
             if self._sum:
                 model[t0:t1, :, c0:c1] += mv.numpy()
             else:
@@ -339,9 +365,15 @@ class ChunkedRimeSolver(object):
                 nxt = stage(jobs[k + 1])      # overlaps this chunk's kernels
             with torch.cuda.stream(copy):
                 copy.wait_event(computed)
-                mv = torch.empty(out['model_vis'].shape, dtype=out['model_vis'].dtype, pin_memory=True)
-                cs = torch.empty(out['chi_sqrd_result'].shape, dtype=out['chi_sqrd_result'].dtype,
-                                 pin_memory=True)
+                if direct:
+                    t0, t1 = job[0], job[1]
+                    mv = torch.from_numpy(model[t0:t1])
+                    cs = torch.from_numpy(chi[t0:t1])
+                else:
+                    mv = torch.empty(out['model_vis'].shape, dtype=out['model_vis'].dtype,
+                                     pin_memory=True)
+                    cs = torch.empty(out['chi_sqrd_result'].shape, dtype=out['chi_sqrd_result'].dtype,
+                                     pin_memory=True)
                 mv.copy_(out['model_vis'], non_blocking=True)
                 cs.copy_(out['chi_sqrd_result'], non_blocking=True)
                 out['model_vis'].record_stream(copy)
@@ -374,9 +406,11 @@ class ChunkedRimeSolver(object):
 
 
 class _HostArrays(object):
-    """Full-size numpy arrays, dims and properties of the v4 schema."""
+    """Full-size numpy arrays, dims and properties of the v4 schema;
+    page-locked (views of pinned torch tensors, kept in _pins) when their
+    total is at most pin_limit bytes."""
 
-    def __init__(self, cfg, ft, ct):
+    def __init__(self, cfg, ft, ct, pin_limit=0):
         from .solver import nr_of_baselines
         autocor = cfg.get(Options.AUTO_CORRELATIONS, False)
         nr = src_types.sources_to_nr_vars(cfg[Options.SOURCES])
@@ -394,12 +428,25 @@ class _HostArrays(object):
         self.dirty = set()          # arrays changed since the last solve
         a1, a2 = np.triu_indices(na, 0 if autocor else 1)
         nbl = self.dims['nbl']
+        specs = []
         for name, shape, dtype, default, _test, scratch in schema.ARRAYS:
             if scratch:
                 continue
-            dt = {'ft': ft, 'ct': ct}.get(dtype, dtype) if isinstance(dtype, str) else dtype
+            dt = np.dtype({'ft': ft, 'ct': ct}.get(dtype, dtype) if isinstance(dtype, str) else dtype)
             shp = tuple(self.dims[s] if isinstance(s, str) else s for s in shape)
-            arr = np.zeros(shp, dtype=dt)
+            specs.append((name, shp, dt, default))
+        total = sum(int(np.prod(shp)) * dt.itemsize for _n, shp, dt, _d in specs)
+        self.pinned = 0 < total <= pin_limit
+        self._pins = {}
+        for name, shp, dt, default in specs:
+            if self.pinned:
+                import torch
+                t = torch.empty(shp, dtype=torch.from_numpy(np.zeros(0, dt)).dtype, pin_memory=True)
+                self._pins[name] = t
+                arr = t.numpy()
+                arr[...] = 0
+            else:
+                arr = np.zeros(shp, dtype=dt)
             if name == 'antenna1':
                 arr[...] = a1[:nbl]
             elif name == 'antenna2':
