@@ -37,6 +37,7 @@
 
 #include <algorithm>
 
+#include "mbx_fmath.hpp"
 #include "mbx_gemm.hpp"
 
 namespace mbx {
@@ -70,12 +71,13 @@ static __host__ __device__ inline size_t gemm_x_offset(int na) {
 }
 
 // Then the two source-record buffers, and in the shaped pass the shape
-// coordinates and kinds of both tiles.
+// coordinates and kinds of both tiles (and in float64 the exp_nonpos table).
 template <typename T, bool SHAPED>
 static inline size_t gemm_lds_bytes(int na, int st) {
     return gemm_x_offset(na) + 2 * size_t(st) * gemm_rows(na) * sizeof(XRow<T>) +
            2 * size_t(st) * sizeof(SrcRec<T>) +
-           (SHAPED ? 2 * 2 * size_t(na) * st * sizeof(T) + 2 * size_t(st) * sizeof(int) : 0);
+           (SHAPED ? 2 * 2 * size_t(na) * st * sizeof(T) + 2 * size_t(st) * sizeof(int) : 0) +
+           (SHAPED && sizeof(T) == 8 ? kExpTab * sizeof(double) : 0);
 }
 
 template <typename T, bool SHAPED>
@@ -152,6 +154,7 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
     T *XYbase = reinterpret_cast<T *>(Pbase + 2 * st);
     const int xystride = 2 * NA * st;        // T per coordinate buffer
     int *Kbase = reinterpret_cast<int *>(XYbase + 2 * xystride);
+    double *Ltab = reinterpret_cast<double *>(Kbase + 2 * st);   // st is a multiple of 4
 
     const int np = dext(d.npsrc), ns = dext(d.nsrc);
     const int sbeg = SHAPED ? np : 0, send = SHAPED ? ns : np;
@@ -177,6 +180,7 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
     }
 
     tile_setup<T>(A, Luvw, Lpa, Lch, t, c, tid, kGemmThreads);
+    if constexpr (SHAPED && sizeof(T) == 8) exp_table_fill(Ltab, tid, kGemmThreads);
     stage_antennas<T>(A, Lant, t, c, tid, kGemmThreads);
     // Jones rows past 2 na read as zero: clear both buffers once (point
     // and shaped tiles write rows < 2 na only)
@@ -273,7 +277,14 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                             // takes the other across.
                             const int p0 = 8 * tpR[j] + 2 * gv;
                             const int pm = min(p0 + (lv & 1), NA - 1);
-                            const T fm = shape_f<T>(gauss, xq - XYs[2 * pm], yq - XYs[2 * pm + 1]);
+                            T fm;
+                            if constexpr (sizeof(T) == 8) {
+                                const T dx = xq - XYs[2 * pm], dy = yq - XYs[2 * pm + 1];
+                                const T r2 = fma(dx, dx, dy * dy);
+                                fm = gauss ? exp_nonpos(-r2, Ltab) : rpow32(1.0 + r2);
+                            } else {
+                                fm = shape_f<T>(gauss, xq - XYs[2 * pm], yq - XYs[2 * pm + 1]);
+                            }
                             const T fo = swap_lane_pairs(fm);
                             const T f0 = (lv & 1) ? fo : fm, f1 = (lv & 1) ? fm : fo;
                             are[j][0] = fma(f0, P[0], are[j][0]); aim[j][0] = fma(f0, Q[0], aim[j][0]);
