@@ -26,6 +26,7 @@
 // mbx_tile.hpp; enter_sample for AdjArgs, mbx_backward.hpp).
 
 #include "mbx_backward.hpp"
+#include "mbx_fmath.hpp"
 #include "mbx_gemm.hpp"
 
 namespace mbx {
@@ -61,7 +62,8 @@ static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, bool shape
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
            + size_t(st) * 4 * sizeof(double)                            // l, m, n - 1
            + size_t(st) * 24 * sizeof(T)                                // shape, I Q U V P, sqrt(B)
-           + (shaped ? 2 * size_t(na) : 0) * sizeof(T);                 // u, v
+           + (shaped ? 2 * size_t(na) : 0) * sizeof(T)                  // u, v
+           + (shaped && sizeof(T) == 8 ? kExpTab * sizeof(double) : 0); // exp_nonpos table
 }
 
 // Gaussian shape coordinates are kept scaled by sqrt(log2 e) in float, so
@@ -91,8 +93,9 @@ static __device__ __forceinline__ float sersic_factor(float r2, float &id) {
     return id * rs;
 }
 static __device__ __forceinline__ double sersic_factor(double r2, double &id) {
-    id = 1.0 / (1.0 + r2);
-    return id * sqrt(id);
+    const double rs = rsq_refined(1.0 + r2);
+    id = rs * rs;
+    return id * rs;
 }
 
 // WIDE (shaped pass only): the items' forward state is not held across the
@@ -157,6 +160,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     T *Lsh = reinterpret_cast<T *>(Lsd + st * 4);                   // [st][8]: shape record
     T *Lst = Lsh + st * 8;                                          // [st][16]: IQUV P . sqrt(B)
     T *Luv = Lst + st * 16;                                         // shaped: (u, v) per antenna
+    double *Ltab = reinterpret_cast<double *>(Luv + 2 * NA);        // shaped float64: exp table
     // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na],
     // written over X once every X read is done (they fit: (kSplit - 1) st na
     // J4 <= st XPS for kSplit = 2, since XPS > 8 na)
@@ -165,6 +169,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
 
     tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kThreads);
     stage_antennas<T>(R, Lant, t, c, tid, kThreads);
+    if constexpr (SHAPED && sizeof(T) == 8) exp_table_fill(Ltab, tid, kThreads);
     for (int i = tid; i < wsize; i += kThreads) Wr[i] = T(0);
     for (int i = tid; i < S.sw * 10; i += kThreads) sacc[i] = 0.0;
     __syncthreads();
@@ -425,7 +430,9 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                         const J4<T> Y = pair_y(q);
                         const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
                         const T dx = xy.x - xp, dy = xy.y - yp;
-                        const T f = gauss_of_scaled(fma(dx, dx, dy * dy));
+                        T f;
+                        if constexpr (sizeof(T) == 8) f = exp_nonpos(-fma(dx, dx, dy * dy), Ltab);
+                        else f = gauss_of_scaled(fma(dx, dx, dy * dy));
                         accumulate(f, Y);
                         moments(f * jdot_re(Jp, Y), dx, dy);
                         advance(q);

@@ -41,15 +41,21 @@ __device__ __forceinline__ double exp_nonpos(double x, const double *tab) {
     return ldexp(p * tab[ki & (kExpTab - 1)], ki >> 6);   // ki = 64 m + j, m = floor(ki / 64)
 }
 
-// d^(-3/2) for d >= 1 (the sersic factor): the hardware reciprocal square
-// root refined by three Newton steps y <- y (3 - d y^2) / 2, each of which
-// squares the relative error, then y^3. d is capped at 1e300 so that d y^2
-// stays finite; the result there underflows to 0 as the exact one does.
-__device__ __forceinline__ double rpow32(double d) {
+// 1 / sqrt(d) for d >= 1: the hardware reciprocal square root refined by
+// three Newton steps y <- y (3 - d y^2) / 2, each of which squares the
+// relative error. d is capped at 1e300 so that d y^2 stays finite (the
+// sersic factors built on it underflow to 0 there, as the exact ones do).
+__device__ __forceinline__ double rsq_refined(double d) {
     d = fmin(d, 1e300);
     double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
     for (int i = 0; i < 3; ++i) y *= fma(-0.5 * d * y, y, 1.5);
+    return y;
+}
+
+// d^(-3/2) for d >= 1 (the sersic factor).
+__device__ __forceinline__ double rpow32(double d) {
+    const double y = rsq_refined(d);
     return y * y * y;
 }
 
