@@ -156,7 +156,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     auto wi_ = [&](int row, int col) { return ((col >> 2) * rows + row) * 4 + (col & 3); };
     J4<T> *Jb = reinterpret_cast<J4<T> *>(Xr); // [st][NA], written over X
     double *sacc = reinterpret_cast<double *>(Xr + xsize);          // [st][10]
-    double *Lsd = sacc + S.sw * 10;                                 // [st][4]: l, m, n - 1
+    double *Lsd = sacc + S.sw * 10;                                 // [st][4]: l, m, n - 1, 1 / n
     T *Lsh = reinterpret_cast<T *>(Lsd + st * 4);                   // [st][8]: shape record
     T *Lst = Lsh + st * 8;                                          // [st][16]: IQUV P . sqrt(B)
     T *Luv = Lst + st * 16;                                         // shaped: (u, v) per antenna
@@ -223,7 +223,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         for (int si = tid; si < nst; si += kThreads) {
             double l0, m0, nm1;
             const J4<T> B = source_sqrtb<T>(R, Lch, t, s0 + si, l0, m0, nm1);
-            Lsd[4 * si] = l0; Lsd[4 * si + 1] = m0; Lsd[4 * si + 2] = nm1;
+            Lsd[4 * si] = l0; Lsd[4 * si + 1] = m0; Lsd[4 * si + 2] = nm1; Lsd[4 * si + 3] = 1.0 / (nm1 + 1.0);
             const T *St = R.stokes + (uint64_t(s0 + si) * NTT + t) * 4;
             T *o = Lst + 16 * si;
             o[0] = St[0]; o[1] = St[1]; o[2] = St[2]; o[3] = St[3];
@@ -268,8 +268,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                                        Lsd[4 * isi + 2], reinterpret_cast<const J4<T> *>(o + 8)[0]);
             else
                 J = item_forward<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[0], an[1], an[2],
-                                    an[3], c, Lch[3], Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], o[0], o[1],
-                                    o[2], o[3], o[4], ist);
+                                    an[3], c, Lch[3], Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], Lsd[4 * isi + 2],
+                                    o[0], o[1], o[2], o[3], o[4], ist);
             T *xrec = Xr + isi * XPS + 10 * ia;
             if (SHAPED) {
 #pragma unroll
@@ -493,13 +493,13 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     const double *an = Lant + 4 * p;
                     const T *o = Lst + 16 * si;
                     (void)item_forward<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[0], an[1], an[2],
-                                          an[3], c, Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1], o[0], o[1],
-                                          o[2], o[3], o[4], ist);
+                                          an[3], c, Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1],
+                                          Lsd[4 * si + 2], o[0], o[1], o[2], o[3], o[4], ist);
                     double g[7];
                     item_backward<T>(ist,
                                      item_chain<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[2], an[3],
-                                                   Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1], ist.okl,
-                                                   ist.okm),
+                                                   Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1],
+                                                   Lsd[4 * si + 3], ist.okl, ist.okm),
                                      acc, g);
 #pragma unroll
                     for (int x = 0; x < 7; ++x) atomicAdd(sac + si * 10 + x, g[x]);
@@ -513,7 +513,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             const double *an = Lant + 4 * ia;
             item_backward<T>(ist,
                              item_chain<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[2], an[3], Lch[3],
-                                           Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], ist.okl, ist.okm),
+                                           Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], Lsd[4 * isi + 3], ist.okl,
+                                           ist.okm),
                              Jb[isi * NA + ia], g);
 #pragma unroll
             for (int x = 0; x < 7; ++x) atomicAdd(sac + isi * 10 + x, g[x]);
@@ -574,7 +575,7 @@ template <typename T> struct AdjPoint {
     static constexpr int kProducers = 512;
     static constexpr int kThreads = kProducers + 64 * kMaxRb;
     static constexpr int kTaps = sizeof(T) == 4 ? 4 : 2;     // beam taps in flight
-    static constexpr int kParams = 16;                       // per source and slot
+    static constexpr int kParams = 17;                       // per source and slot
     // which role stages the next tile's parameters: in float64 the matrix
     // waves (registers to spare), in float the producers (the matrix
     // waves' W leaves no room)
@@ -647,7 +648,7 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
     auto tile_n = [&](int it) { return min(st, np - it * st); };
 
     // parameters of tile `it` by the first matrix wave: l, m, n - 1, I, Q,
-    // U, V, P = (f / f_ref)^alpha, sqrt(B) (8 reals)
+    // U, V, P = (f / f_ref)^alpha, sqrt(B) (8 reals), 1 / n
     auto stage_params = [&](int it, int si) {
         if (it >= ntiles || si >= tile_n(it)) return;
         const int s = it * st + si;
@@ -660,6 +661,7 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
         v[7] = exp_t<T>(double(R.alpha[uint64_t(s) * NT + t]) * Lch[4]);
 #pragma unroll
         for (int x = 0; x < 4; ++x) { v[8 + 2 * x] = B.v[x].re; v[9 + 2 * x] = B.v[x].im; }
+        v[16] = 1.0 / (nm1 + 1.0);
 #pragma unroll
         for (int f = 0; f < C::kParams; ++f) prm(it, f)[si] = v[f];
     };
@@ -727,12 +729,12 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
                     ItemState<T> ist;
                     (void)item_forward<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2],
                                           an[3], c, Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si],
-                                          T(prm(it, 3)[si]), T(prm(it, 4)[si]), T(prm(it, 5)[si]),
-                                          T(prm(it, 6)[si]), T(prm(it, 7)[si]), ist);
+                                          prm(it, 2)[si], T(prm(it, 3)[si]), T(prm(it, 4)[si]),
+                                          T(prm(it, 5)[si]), T(prm(it, 6)[si]), T(prm(it, 7)[si]), ist);
                     item_backward<T>(ist,
                                      item_chain<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[2], an[3],
-                                                   Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si], ist.okl,
-                                                   ist.okm),
+                                                   Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si],
+                                                   prm(it, 16)[si], ist.okl, ist.okm),
                                      Jb, gr);
                     double *sa = sacc + ((it / S.wt) & 1) * wsrc * 7 + ((it % S.wt) * st + si) * 7;
 #pragma unroll

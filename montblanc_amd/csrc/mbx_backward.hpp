@@ -95,15 +95,16 @@ struct ItemChain {
     double lnr, dphi_dl, dphi_dm, dvl_dl, dvl_dm, dvm_dl, dvm_dm;
 };
 
+// rn = 1 / n, n = sqrt(1 - l0^2 - m0^2): a per-source value the caller
+// keeps, so that no item evaluates a square root or a division for it.
 template <typename T>
 __device__ __forceinline__ ItemChain item_chain(const RimeArgs<T> &A, const double *xa, double ps,
                                                 double pc, double al, double am, double f, double lnr,
-                                                double l0, double m0, bool okl, bool okm) {
-    const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
+                                                double l0, double m0, double rn, bool okl, bool okm) {
     ItemChain ch;
     ch.lnr = lnr;
-    ch.dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * l0 / n);
-    ch.dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * m0 / n);
+    ch.dphi_dl = -A.two_pi_over_c * f * (xa[0] - xa[2] * (l0 * rn));
+    ch.dphi_dm = -A.two_pi_over_c * f * (xa[1] - xa[2] * (m0 * rn));
     ch.dvl_dl = okl ? A.lscale * al * pc : 0.0; ch.dvl_dm = okl ? -A.lscale * al * ps : 0.0;
     ch.dvm_dl = okm ? A.mscale * am * ps : 0.0; ch.dvm_dm = okm ? A.mscale * am * pc : 0.0;
     return ch;
@@ -114,15 +115,15 @@ __device__ __forceinline__ ItemChain item_chain(const RimeArgs<T> &A, const doub
 // P = (f/f_ref)^alpha at this (time, channel); the antenna's uvw (xa),
 // parallactic sin/cos (ps, pc), pointing error (pel, pem) and scaling
 // (al, am); the channel c (beam plane), its frequency f and
-// lnr = log(f/f_ref). Returns J = E K sqrt(B).
+// lnr = log(f/f_ref); nm1 = n - 1 of the source (n = sqrt(1 - l0^2 - m0^2),
+// computed once per source by the caller). Returns J = E K sqrt(B).
 template <typename T>
 __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double *xa, double ps,
                                               double pc, double pel, double pem, double al,
                                               double am, int c, double f, double lnr, double l0,
-                                              double m0, T I, T Q, T U, T V, T P, ItemState<T> &st) {
+                                              double m0, double nm1, T I, T Q, T U, T V, T P,
+                                              ItemState<T> &st) {
     const mbx_dims &d = A.d;
-    const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
-    const double nm1 = n - 1.0;
     // ---- beam value and its derivatives along the grid fractions
     const double lr = (l0 * pc - m0 * ps + pel) * al;
     const double mr = (l0 * ps + m0 * pc + pem) * am;
@@ -274,8 +275,10 @@ __device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const 
                                                     T U, T V, T P, const J4<T> &Jb,
                                                     double *g /* [7] */) {
     ItemState<T> st;
-    (void)item_forward<T>(A, xa, ps, pc, pel, pem, al, am, c, f, lnr, l0, m0, I, Q, U, V, P, st);
-    item_backward<T>(st, item_chain<T>(A, xa, ps, pc, al, am, f, lnr, l0, m0, st.okl, st.okm), Jb, g);
+    const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
+    (void)item_forward<T>(A, xa, ps, pc, pel, pem, al, am, c, f, lnr, l0, m0, n - 1.0, I, Q, U, V, P, st);
+    item_backward<T>(st, item_chain<T>(A, xa, ps, pc, al, am, f, lnr, l0, m0, 1.0 / n, st.okl, st.okm), Jb,
+                     g);
 }
 
 // The same item reading its inputs from global memory and the tile's
