@@ -25,6 +25,8 @@
 // instances cover several samples' grids in one launch (SampleSpan,
 // mbx_tile.hpp; enter_sample for AdjArgs, mbx_backward.hpp).
 
+#include <type_traits>
+
 #include "mbx_backward.hpp"
 #include "mbx_fmath.hpp"
 #include "mbx_gemm.hpp"
@@ -60,8 +62,8 @@ static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, bool shape
     const size_t x = shaped ? size_t(st) * adjs_xps(na) : 2 * size_t(2 * st) * rows; // X (then Jb)
     return adjg_setup_bytes(na) + (w + x) * sizeof(T)
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
-           + size_t(st) * 4 * sizeof(double)                            // l, m, n - 1
-           + size_t(st) * 24 * sizeof(T)                                // shape, I Q U V P, sqrt(B)
+           + 2 * size_t(st) * 4 * sizeof(double)                        // l, m, n - 1, 1 / n (2 tiles)
+           + 2 * size_t(st) * 24 * sizeof(T)                            // shape, I Q U V P, sqrt(B)
            + (shaped ? 2 * size_t(na) : 0) * sizeof(T)                  // u, v
            + (shaped && sizeof(T) == 8 ? kExpTab * sizeof(double) : 0); // exp_nonpos table
 }
@@ -156,10 +158,12 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     auto wi_ = [&](int row, int col) { return ((col >> 2) * rows + row) * 4 + (col & 3); };
     J4<T> *Jb = reinterpret_cast<J4<T> *>(Xr); // [st][NA], written over X
     double *sacc = reinterpret_cast<double *>(Xr + xsize);          // [st][10]
-    double *Lsd = sacc + S.sw * 10;                                 // [st][4]: l, m, n - 1, 1 / n
-    T *Lsh = reinterpret_cast<T *>(Lsd + st * 4);                   // [st][8]: shape record
-    T *Lst = Lsh + st * 8;                                          // [st][16]: IQUV P . sqrt(B)
-    T *Luv = Lst + st * 16;                                         // shaped: (u, v) per antenna
+    // per-source records of two tiles (the next one is staged during the
+    // current one's Jones production)
+    double *Lsd0 = sacc + S.sw * 10;                                // [2][st][4]: l, m, n - 1, 1 / n
+    T *Lsh0 = reinterpret_cast<T *>(Lsd0 + 2 * st * 4);             // [2][st][8]: shape record
+    T *Lst0 = Lsh0 + 2 * st * 8;                                    // [2][st][16]: IQUV P . sqrt(B)
+    T *Luv = Lst0 + 2 * st * 16;                                    // shaped: (u, v) per antenna
     double *Ltab = reinterpret_cast<double *>(Luv + 2 * NA);        // shaped float64: exp table
     // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na],
     // written over X once every X read is done (they fit: (kSplit - 1) st na
@@ -215,41 +219,52 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     // Each source's gradient is summed in LDS over a window of sw sources
     // and added to global memory once per window: the float64 atomics'
     // latency is paid once per window, not once per tile.
-    int wbase = sbeg;
-    for (int s0 = sbeg; s0 < send; s0 += st) {
-        const int nst = min(st, send - s0);
-        double *sac = sacc + (s0 - wbase) * 10;    // this tile's sums
-        // ---- the tile's source parameters, once
-        for (int si = tid; si < nst; si += kThreads) {
-            double l0, m0, nm1;
-            const J4<T> B = source_sqrtb<T>(R, Lch, t, s0 + si, l0, m0, nm1);
-            Lsd[4 * si] = l0; Lsd[4 * si + 1] = m0; Lsd[4 * si + 2] = nm1; Lsd[4 * si + 3] = 1.0 / (nm1 + 1.0);
-            const T *St = R.stokes + (uint64_t(s0 + si) * NTT + t) * 4;
-            T *o = Lst + 16 * si;
-            o[0] = St[0]; o[1] = St[1]; o[2] = St[2]; o[3] = St[3];
-            o[4] = exp_t<T>(double(R.alpha[uint64_t(s0 + si) * NTT + t]) * Lch[4]);
-            reinterpret_cast<J4<T> *>(o + 8)[0] = B;
-            if (SHAPED) {
-                // shape record: gaussian 1, k, k R, el, em, R (k = gauss_scale f);
-                // sersic 2, k R', 2 e1 / den, 2 e2 / den, k / den, e1, e2
-                // (k = 2 pi f / C, den = 1 - e1^2 - e2^2, R' = R / den)
-                const int s = s0 + si;
-                T *h = Lsh + 8 * si;
-                if (s < np + ng) {
-                    const int g = s - np;
-                    const double el = R.gauss[g], em = R.gauss[g + NGS], Rg = R.gauss[g + 2 * NGS];
-                    const double k = R.gauss_scale * Lch[3];
-                    h[0] = T(1); h[1] = T(k); h[2] = T(k * Rg); h[3] = T(el); h[4] = T(em); h[5] = T(Rg);
-                } else {
-                    const int g = s - np - ng;
-                    const double e1 = R.sersic[g], e2 = R.sersic[g + NSS], Rs = R.sersic[g + 2 * NSS];
-                    const double den = 1.0 - e1 * e1 - e2 * e2, k = R.two_pi_over_c * Lch[3];
-                    h[0] = T(2); h[1] = T(k * Rs / den); h[2] = T(2.0 * e1 / den); h[3] = T(2.0 * e2 / den);
-                    h[4] = T(k / den); h[5] = T(e1); h[6] = T(e2);
-                }
+    // source records of the tile at s0t (ns sources) into buffer buf, one
+    // thread per source from thread first on
+    auto stage_params = [&](int s0t, int ns, int buf, int first) {
+        const int si = tid - first;
+        if (si < 0 || si >= ns) return;
+        double *Lsd = Lsd0 + buf * st * 4;
+        T *Lsh = Lsh0 + buf * st * 8, *Lst = Lst0 + buf * st * 16;
+        double l0, m0, nm1;
+        const J4<T> B = source_sqrtb<T>(R, Lch, t, s0t + si, l0, m0, nm1);
+        Lsd[4 * si] = l0; Lsd[4 * si + 1] = m0; Lsd[4 * si + 2] = nm1; Lsd[4 * si + 3] = 1.0 / (nm1 + 1.0);
+        const T *St = R.stokes + (uint64_t(s0t + si) * NTT + t) * 4;
+        T *o = Lst + 16 * si;
+        o[0] = St[0]; o[1] = St[1]; o[2] = St[2]; o[3] = St[3];
+        o[4] = exp_t<T>(double(R.alpha[uint64_t(s0t + si) * NTT + t]) * Lch[4]);
+        reinterpret_cast<J4<T> *>(o + 8)[0] = B;
+        if (SHAPED) {
+            // shape record: gaussian 1, k, k R, el, em, R (k = gauss_scale f);
+            // sersic 2, k R', 2 e1 / den, 2 e2 / den, k / den, e1, e2
+            // (k = 2 pi f / C, den = 1 - e1^2 - e2^2, R' = R / den)
+            const int s = s0t + si;
+            T *h = Lsh + 8 * si;
+            if (s < np + ng) {
+                const int g = s - np;
+                const double el = R.gauss[g], em = R.gauss[g + NGS], Rg = R.gauss[g + 2 * NGS];
+                const double k = R.gauss_scale * Lch[3];
+                h[0] = T(1); h[1] = T(k); h[2] = T(k * Rg); h[3] = T(el); h[4] = T(em); h[5] = T(Rg);
+            } else {
+                const int g = s - np - ng;
+                const double e1 = R.sersic[g], e2 = R.sersic[g + NSS], Rs = R.sersic[g + 2 * NSS];
+                const double den = 1.0 - e1 * e1 - e2 * e2, k = R.two_pi_over_c * Lch[3];
+                h[0] = T(2); h[1] = T(k * Rs / den); h[2] = T(2.0 * e1 / den); h[3] = T(2.0 * e2 / den);
+                h[4] = T(k / den); h[5] = T(e1); h[6] = T(e2);
             }
         }
-        __syncthreads();
+    };
+    if (sbeg < send) stage_params(sbeg, min(st, send - sbeg), 0, 0);
+    __syncthreads();
+    int wbase = sbeg;
+    for (int s0 = sbeg, it = 0; s0 < send; s0 += st, ++it) {
+        const int nst = min(st, send - s0);
+        double *sac = sacc + (s0 - wbase) * 10;    // this tile's sums
+        const double *Lsd = Lsd0 + (it & 1) * st * 4;
+        const T *Lsh = Lsh0 + (it & 1) * st * 8, *Lst = Lst0 + (it & 1) * st * 16;
+        // the next tile's source records, by the last st threads (idle in
+        // the Jones production below when st na < threads)
+        if (s0 + st < send) stage_params(s0 + st, min(st, send - s0 - st), (it + 1) & 1, kThreads - st);
         // ---- X of the tile (columns of sources past nst are zero). Each
         // thread owns one (source, antenna) item (st * na <= threads) and
         // keeps its forward state for the backward after the product.
@@ -362,20 +377,28 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             }
         } else {
             // one (source, antenna p) per thread: st * NA <= threads. The
-            // shape-factor adjoint of a pair is the same in both orders
-            // (f and its derivatives are even in the pair's (u, v), and
-            // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): every thread sums it
-            // over all its q and the combination takes half, which keeps the
-            // pair loop free of a branch that diverges inside every wave
-            // (a wave holds several antennas p).
+            // shape-factor adjoint of a pair is the same in both orders (f
+            // and its derivatives are even in the pair's (u, v), and
+            // Re <J_p, W_pq J_q> = Re <J_q, W_qp J_p>): it is summed over the
+            // pairs q > p only and counted twice. So that the pair loop has
+            // no branch that diverges inside a wave (a wave holds several
+            // antennas p), q runs in three segments bounded by the wave's
+            // smallest and largest p: q below every lane's p (no moments),
+            // q among the wave's antennas (moments weighted by q > p), q
+            // above every lane's p (moments).
             const int part = min(tid / (nst * NA), kSplit - 1);
             const int r = tid - part * nst * NA;
             const bool mine = r < nst * NA;
             const int si = mine ? r % nst : 0, p = mine ? r / nst : 0;
             // pair range: all antennas, or the part-th of kSplit ranges (WIDE)
             const int q0 = part * NA / kSplit, q1 = (part + 1) * NA / kSplit;
-            // two pairs in flight in float (registers to spare), one in float64
-            constexpr int kQUnroll = sizeof(T) == 4 ? 2 : 1;
+            int plo = mine ? p : NA, phi = mine ? p : -1;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                plo = min(plo, __shfl_xor(plo, o));
+                phi = max(phi, __shfl_xor(phi, o));
+            }
+            const int qa = max(q0, min(q1, plo)), qb = max(qa, min(q1, phi + 1));
             J4<T> acc = jzero<T>();
             T sh0 = T(0), sh1 = T(0), sh2 = T(0);
             if (mine) {
@@ -391,10 +414,10 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 const T *wq = Wr + p * WPS + 8 * q0;
                 const T *xq = xs + 10 * q0;
                 const T *uq = Luv + 2 * q0;
-                auto pair_y = [&](int) {
+                auto pair_y = [&]() {
                     return jmul(jload(reinterpret_cast<const cx<T> *>(wq)), jload_pairs(xq));
                 };
-                auto advance = [&](int) {
+                auto advance = [&]() {
                     wq += 8;
                     xq += 10;
                     uq += 2;
@@ -409,8 +432,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 // Each pair's shape-factor adjoint is t_q = f Re <J_p, W_pq J_q>
                 // (D = 1 + r2 for sersic: t_q = f Re <..> / D) times a form
                 // linear in (dx du, dx dv, dy du, dy dv) and, for sersic, r2:
-                // those moments are summed over q and combined once.
-                T m_xu = T(0), m_xv = T(0), m_yu = T(0), m_yv = T(0);
+                // those moments are summed over q > p and combined once.
+                T m_xu = T(0), m_xv = T(0), m_yu = T(0), m_yv = T(0), m_r = T(0);
                 auto moments = [&](T tq, T dx, T dy) {
                     const v2 uv = *reinterpret_cast<const v2 *>(uq);
                     const T du = uv.x - up, dv = uv.y - vp;
@@ -420,51 +443,66 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     m_yu = fma(ty, du, m_yu);
                     m_yv = fma(ty, dv, m_yv);
                 };
+                // one pair of kind GAUSS; MOM 0: no moments, 1: moments
+                // weighted by q > p, 2: moments
+                auto pair = [&](auto gauss_tag, auto mom_tag, int q) {
+                    constexpr bool GAUSS = decltype(gauss_tag)::value;
+                    constexpr int MOM = decltype(mom_tag)::value;
+                    const J4<T> Y = pair_y();
+                    const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
+                    const T dx = xy.x - xp, dy = xy.y - yp;
+                    const T r2 = fma(dx, dx, dy * dy);
+                    T f, id = T(1);
+                    if constexpr (GAUSS) {
+                        if constexpr (sizeof(T) == 8) f = exp_nonpos(-r2, Ltab);
+                        else f = gauss_of_scaled(r2);
+                    } else {
+                        f = sersic_factor(r2, id);
+                    }
+                    accumulate(f, Y);
+                    if constexpr (MOM > 0) {
+                        T tq = f * jdot_re(Jp, Y);
+                        if constexpr (!GAUSS) tq *= id;
+                        if constexpr (MOM == 1) tq = q > p ? tq : T(0);
+                        moments(tq, dx, dy);
+                        if constexpr (!GAUSS) m_r = fma(tq, r2, m_r);
+                    }
+                    advance();
+                };
+                // two pairs in flight in float (registers to spare), one in
+                // float64
+                auto walk = [&](auto gauss_tag) {
+                    typedef std::integral_constant<int, 0> M0;
+                    typedef std::integral_constant<int, 1> M1;
+                    typedef std::integral_constant<int, 2> M2;
+#pragma unroll (sizeof(T) == 4 ? 2 : 1)
+                    for (int q = q0; q < qa; ++q) pair(gauss_tag, M0(), q);
+                    for (int q = qa; q < qb; ++q) pair(gauss_tag, M1(), q);
+#pragma unroll (sizeof(T) == 4 ? 2 : 1)
+                    for (int q = qb; q < q1; ++q) pair(gauss_tag, M2(), q);
+                };
                 if (h[0] == T(1)) {
+                    walk(std::true_type());
                     // gaussian: f = exp(-r2); d f / d(el, em, R) = -2 f k
                     // (dy du - R dx dv, R dx du + dy dv, dx (du em - dv el));
-                    // dx, dy here are scaled by g = adj_gauss_pre
+                    // dx, dy here are scaled by g = adj_gauss_pre; x 2 for
+                    // the two orders of each pair
                     const T k = h[1], el = h[3], em = h[4], Rg = h[5];
-#pragma unroll kQUnroll
-                    for (int q = q0; q < q1; ++q) {
-                        const J4<T> Y = pair_y(q);
-                        const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
-                        const T dx = xy.x - xp, dy = xy.y - yp;
-                        T f;
-                        if constexpr (sizeof(T) == 8) f = exp_nonpos(-fma(dx, dx, dy * dy), Ltab);
-                        else f = gauss_of_scaled(fma(dx, dx, dy * dy));
-                        accumulate(f, Y);
-                        moments(f * jdot_re(Jp, Y), dx, dy);
-                        advance(q);
-                    }
-                    const T c4 = T(-2) * k / T(adj_gauss_pre<T>());
+                    const T c4 = T(-4) * k / T(adj_gauss_pre<T>());
                     sh0 = c4 * (m_yu - Rg * m_xv);
                     sh1 = c4 * fma(Rg, m_xu, m_yv);
                     sh2 = c4 * (em * m_xu - el * m_xv);
                 } else {
+                    walk(std::false_type());
                     // sersic: D = 1 + r2, f = D^-3/2, c = -3 f / D; d f / d(e1,
                     // e2, R) = c (kR' (dx du - dy dv) + a1 r2, kR' (dx dv + dy du)
                     // + a2 r2, k/den (dx dx0 + dy dy0)) with dx0 = du (1 + e1)
-                    // + dv e2, dy0 = du e2 + dv (1 - e1)
+                    // + dv e2, dy0 = du e2 + dv (1 - e1); x 2 for the two
+                    // orders of each pair
                     const T kR = h[1], a1 = h[2], a2 = h[3], kd = h[4], e1 = h[5], e2 = h[6];
-                    T m_r = T(0);
-#pragma unroll kQUnroll
-                    for (int q = q0; q < q1; ++q) {
-                        const J4<T> Y = pair_y(q);
-                        const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
-                        const T dx = xy.x - xp, dy = xy.y - yp;
-                        const T r2 = fma(dx, dx, dy * dy);
-                        T id;
-                        const T f = sersic_factor(r2, id);
-                        accumulate(f, Y);
-                        const T tq = f * id * jdot_re(Jp, Y);
-                        moments(tq, dx, dy);
-                        m_r = fma(tq, r2, m_r);
-                        advance(q);
-                    }
-                    sh0 = T(-3) * fma(kR, m_xu - m_yv, a1 * m_r);
-                    sh1 = T(-3) * fma(kR, m_xv + m_yu, a2 * m_r);
-                    sh2 = T(-3) * kd * ((T(1) + e1) * m_xu + e2 * (m_xv + m_yu) + (T(1) - e1) * m_yv);
+                    sh0 = T(-6) * fma(kR, m_xu - m_yv, a1 * m_r);
+                    sh1 = T(-6) * fma(kR, m_xv + m_yu, a2 * m_r);
+                    sh2 = T(-6) * kd * ((T(1) + e1) * m_xu + e2 * (m_xv + m_yu) + (T(1) - e1) * m_yv);
                 }
             }
             __syncthreads();                       // every X read done
