@@ -49,16 +49,18 @@ static __host__ __device__ inline size_t adjg_setup_bytes(int na) {
     return ((sizeof(double) * (9 * size_t(na) + CH)) + 31) & ~size_t(31);
 }
 
-// Shaped pass record strides (elements). W block (p, q) sits at
-// p WPS + 8 q, X record (s, a) at s XPS + 10 a; the row padding puts the
-// rows of the eight antennas (W) or sources (X) one wave reads at once in
-// different LDS banks (WPS / 4 and XPS / 2 odd).
-__host__ __device__ inline int adjs_wps(int na) { return 8 * na + 4; }
+// Shaped pass records (elements). W is kept as its upper triangle: block
+// (a, b), a <= b, at adjs_wrow(a) + 8 (b - a), rows padded by four values
+// so that the rows of the antennas one wave reads at once start in
+// different LDS banks; W_pq for q < p is the conjugate transpose of block
+// (q, p). X record (s, a) at s XPS + 10 a, XPS / 2 odd for the same reason.
+__host__ __device__ inline int adjs_wrow(int a, int na) { return a * (8 * na + 4) - 4 * a * (a - 1); }
+__host__ __device__ inline int adjs_welems(int na) { return 4 * na * (na + 2); }
 __host__ __device__ inline int adjs_xps(int na) { return 10 * na + ((na & 1) ? 4 : 2); }
 
 template <typename T>
 static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, bool shaped) {
-    const size_t w = shaped ? size_t(na) * adjs_wps(na) : 2 * size_t(rows) * rows;   // W
+    const size_t w = shaped ? size_t(adjs_welems(na)) : 2 * size_t(rows) * rows;      // W
     const size_t x = shaped ? size_t(st) * adjs_xps(na) : 2 * size_t(2 * st) * rows; // X (then Jb)
     return adjg_setup_bytes(na) + (w + x) * sizeof(T)
            + size_t(sw) * 10 * sizeof(double)                           // gradient sums
@@ -102,14 +104,11 @@ static __device__ __forceinline__ double sersic_factor(double r2, double &id) {
 
 // WIDE (shaped pass only): the items' forward state is not held across the
 // pair loop but recomputed for the backward, which leaves registers for
-// more waves; each (source, antenna p) item's pair loop is split over
-// several threads (q in consecutive ranges of the antennas), whose partial
-// Jones adjoints meet in LDS.
-// Float only: 1024 threads, each item's pair loop split over two. (In
-// float64 the recomputed backward needs more registers than sixteen, or
-// twelve, waves leave; measured slower than keeping the state.)
+// more waves: 1024 threads, one (source, antenna p) item each, and the
+// item's thread runs its backward right after its pair loop. Float only
+// (in float64 the recomputed backward needs more registers than sixteen,
+// or twelve, waves leave; measured slower than keeping the state).
 template <typename T, bool WIDE> constexpr int adjg_threads() { return WIDE ? 1024 : kAdjThreads; }
-template <typename T, bool WIDE> constexpr int adjg_split() { return WIDE ? 2 : 1; }
 
 template <typename T, bool SHAPED, bool WIDE = false, bool BATCHED = false>
 __global__ void __launch_bounds__((adjg_threads<T, WIDE>()))
@@ -117,7 +116,6 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     static_assert(SHAPED || !WIDE, "the wide layout is for the shaped pass");
     static_assert(sizeof(T) == 4 || !WIDE, "the wide layout is for float");
     constexpr int kThreads = adjg_threads<T, WIDE>();
-    constexpr int kSplit = adjg_split<T, WIDE>();
     extern __shared__ __align__(32) unsigned char smem_raw[];
     // in a launch over several samples: this workgroup's sample and slot
     const SampleAt at = enter_sample_if<BATCHED>(A);
@@ -139,11 +137,12 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     // Point pass: W and X as separate re and im planes; X is [n][row] (MFMA
     // operand reads along the rows), n = 2 s + j, row = 2 a + k.
     // Shaped pass: complex interleaved records, walked by pointers over q:
-    //   W block (p, q) at Wr + p WPS + 8 q: W_pq[i][k] at 2 (2 i + k) (re, im);
+    //   W block (a, b), a <= b, at Wr + adjs_wrow(a) + 8 (b - a): W_ab[i][k]
+    //   at 2 (2 i + k) (re, im);
     //   X record (s, a) at Xr + s XPS + 10 a: J_as[k][j] at 2 (2 k + j)
     //   (re, im), then the shape coordinates x, y of (antenna a, source s).
-    const int WPS = adjs_wps(NA), XPS = adjs_xps(NA);
-    const int wsize = SHAPED ? NA * WPS : 2 * rows * rows;
+    const int XPS = adjs_xps(NA);
+    const int wsize = SHAPED ? adjs_welems(NA) : 2 * rows * rows;
     const int xsize = SHAPED ? st * XPS : 4 * st * rows;
     T *Wr = reinterpret_cast<T *>(smem_raw + adjg_setup_bytes(NA));
     T *Wi = Wr + rows * rows;
@@ -165,11 +164,6 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     T *Lst0 = Lsh0 + 2 * st * 8;                                    // [2][st][16]: IQUV P . sqrt(B)
     T *Luv = Lst0 + 2 * st * 16;                                    // shaped: (u, v) per antenna
     double *Ltab = reinterpret_cast<double *>(Luv + 2 * NA);        // shaped float64: exp table
-    // WIDE: partial Jones adjoints of the later pair ranges [kSplit - 1][st][na],
-    // written over X once every X read is done (they fit: (kSplit - 1) st na
-    // J4 <= st XPS for kSplit = 2, since XPS > 8 na)
-    static_assert(kSplit <= 2, "the partials are kept over X");
-    J4<T> *Jpart = Jb;
 
     tile_setup<T>(R, Luvw, Lpa, Lch, t, c, tid, kThreads);
     stage_antennas<T>(R, Lant, t, c, tid, kThreads);
@@ -183,7 +177,8 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     // (re, im) of W_pq[i][k]: point pass planes, shaped pass records
     auto wput = [&](int p, int i, int q, int k, T re, T im) {
         if (SHAPED) {
-            T *w = Wr + p * WPS + 8 * q + 2 * (2 * i + k);
+            if (p > q) return;                 // the upper triangle only
+            T *w = Wr + adjs_wrow(p, NA) + 8 * (q - p) + 2 * (2 * i + k);
             w[0] = re;
             w[1] = im;
         } else {
@@ -386,19 +381,15 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             // smallest and largest p: q below every lane's p (no moments),
             // q among the wave's antennas (moments weighted by q > p), q
             // above every lane's p (moments).
-            const int part = min(tid / (nst * NA), kSplit - 1);
-            const int r = tid - part * nst * NA;
-            const bool mine = r < nst * NA;
-            const int si = mine ? r % nst : 0, p = mine ? r / nst : 0;
-            // pair range: all antennas, or the part-th of kSplit ranges (WIDE)
-            const int q0 = part * NA / kSplit, q1 = (part + 1) * NA / kSplit;
+            const bool mine = tid < nst * NA;
+            const int si = mine ? tid % nst : 0, p = mine ? tid / nst : 0;
             int plo = mine ? p : NA, phi = mine ? p : -1;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 plo = min(plo, __shfl_xor(plo, o));
                 phi = max(phi, __shfl_xor(phi, o));
             }
-            const int qa = max(q0, min(q1, plo)), qb = max(qa, min(q1, phi + 1));
+            const int qa = min(NA, plo), qb = max(qa, min(NA, phi + 1));
             J4<T> acc = jzero<T>();
             T sh0 = T(0), sh1 = T(0), sh2 = T(0);
             if (mine) {
@@ -409,16 +400,11 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 const v2 xyp = *reinterpret_cast<const v2 *>(xs + 10 * p + 8);
                 const v2 uvp = *reinterpret_cast<const v2 *>(Luv + 2 * p);
                 const T xp = xyp.x, yp = xyp.y, up = uvp.x, vp = uvp.y;
-                // pair q's W block, X record and (u, v): one running pointer
-                // each, fixed offsets within a record
-                const T *wq = Wr + p * WPS + 8 * q0;
-                const T *xq = xs + 10 * q0;
-                const T *uq = Luv + 2 * q0;
-                auto pair_y = [&]() {
-                    return jmul(jload(reinterpret_cast<const cx<T> *>(wq)), jload_pairs(xq));
-                };
+                // pair q's X record and (u, v): one running pointer each,
+                // fixed offsets within a record (q = 0, 1, ... in order)
+                const T *xq = xs;
+                const T *uq = Luv;
                 auto advance = [&]() {
-                    wq += 8;
                     xq += 10;
                     uq += 2;
                 };
@@ -443,12 +429,12 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     m_yu = fma(ty, du, m_yu);
                     m_yv = fma(ty, dv, m_yv);
                 };
-                // one pair of kind GAUSS; MOM 0: no moments, 1: moments
-                // weighted by q > p, 2: moments
-                auto pair = [&](auto gauss_tag, auto mom_tag, int q) {
+                // one pair (W = W_pq) of kind GAUSS; MOM 0: no moments, 1:
+                // moments weighted by q > p, 2: moments
+                auto pair = [&](auto gauss_tag, auto mom_tag, int q, const J4<T> &W) {
                     constexpr bool GAUSS = decltype(gauss_tag)::value;
                     constexpr int MOM = decltype(mom_tag)::value;
-                    const J4<T> Y = pair_y();
+                    const J4<T> Y = jmul(W, jload_pairs(xq));
                     const v2 xy = *reinterpret_cast<const v2 *>(xq + 8);
                     const T dx = xy.x - xp, dy = xy.y - yp;
                     const T r2 = fma(dx, dx, dy * dy);
@@ -471,15 +457,34 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                 };
                 // two pairs in flight in float (registers to spare), one in
                 // float64
+                auto wblock = [&](const T *w) { return jload(reinterpret_cast<const cx<T> *>(w)); };
                 auto walk = [&](auto gauss_tag) {
                     typedef std::integral_constant<int, 0> M0;
                     typedef std::integral_constant<int, 1> M1;
                     typedef std::integral_constant<int, 2> M2;
+                    // q below the wave's antennas: W_pq = (block (q, p))^H,
+                    // down column p of the triangle
+                    const T *wc = Wr + 8 * p;           // block (0, p)
+                    int step = 8 * NA - 4;              // to block (q + 1, p)
 #pragma unroll (sizeof(T) == 4 ? 2 : 1)
-                    for (int q = q0; q < qa; ++q) pair(gauss_tag, M0(), q);
-                    for (int q = qa; q < qb; ++q) pair(gauss_tag, M1(), q);
+                    for (int q = 0; q < qa; ++q) {
+                        pair(gauss_tag, M0(), q, jherm(wblock(wc)));
+                        wc += step;
+                        step -= 8;
+                    }
+                    // q among the wave's antennas: either block, per lane
+                    for (int q = qa; q < qb; ++q) {
+                        const int lo = min(p, q), hi = max(p, q);
+                        const J4<T> B = wblock(Wr + adjs_wrow(lo, NA) + 8 * (hi - lo));
+                        pair(gauss_tag, M1(), q, q < p ? jherm(B) : B);
+                    }
+                    // q above the wave's antennas: block (p, q) along row p
+                    const T *wr = Wr + adjs_wrow(p, NA) + 8 * (qb - p);
 #pragma unroll (sizeof(T) == 4 ? 2 : 1)
-                    for (int q = qb; q < q1; ++q) pair(gauss_tag, M2(), q);
+                    for (int q = qb; q < NA; ++q) {
+                        pair(gauss_tag, M2(), q, wblock(wr));
+                        wr += 8;
+                    }
                 };
                 if (h[0] == T(1)) {
                     walk(std::true_type());
@@ -505,29 +510,14 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     sh2 = T(-6) * kd * ((T(1) + e1) * m_xu + e2 * (m_xv + m_yu) + (T(1) - e1) * m_yv);
                 }
             }
-            __syncthreads();                       // every X read done
+            if (!WIDE) __syncthreads();            // every X read done: Jb goes over X
             if (mine) {
                 if (!WIDE) Jb[si * NA + p] = acc;
-                else if (part) Jpart[((part - 1) * nst + si) * NA + p] = acc;
                 atomicAdd(sac + si * 10 + 7, double(sh0));
                 atomicAdd(sac + si * 10 + 8, double(sh1));
                 atomicAdd(sac + si * 10 + 9, double(sh2));
-            }
-            if (WIDE) {
-                // the first range's threads own the items: add the other
-                // ranges' partials and run the backward with the forward
-                // state recomputed
-                __syncthreads();
-                if (mine && !part) {
-#pragma unroll
-                    for (int h = 1; h < kSplit; ++h) {
-                        const J4<T> Jo = Jpart[((h - 1) * nst + si) * NA + p];
-#pragma unroll
-                        for (int x = 0; x < 4; ++x) {
-                            acc.v[x].re += Jo.v[x].re;
-                            acc.v[x].im += Jo.v[x].im;
-                        }
-                    }
+                if (WIDE) {
+                    // the item's backward, with the forward state recomputed
                     const double *an = Lant + 4 * p;
                     const T *o = Lst + 16 * si;
                     (void)item_forward<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[0], an[1], an[2],
@@ -893,23 +883,26 @@ static bool adjp_shape(const mbx_dims &d, AdjPointShape &S, size_t &lds) {
     return true;
 }
 
+// Tile of k_adj_gemm: one (source, antenna) item per thread, st sources
+// (a multiple of 8; at most 16 in the point pass, whose product tiles it
+// bounds, 32 in the shaped pass), the most that fit the LDS budget; then
+// the widest gradient window up to 32 tiles.
 template <typename T>
 static bool adjg_shape(const mbx_dims &d, AdjGemmShape &S, size_t &lds, bool shaped,
-                       int threads = kAdjThreads, int split = 1) {
+                       int threads = kAdjThreads) {
     const int na = d.na.local_size;
     if (na <= 0 || na > 64) return false;
     S.rows = ((2 * na + 15) / 16) * 16;
-    S.st = std::min(16, (threads / split / na) / 8 * 8);
-    if (S.st < 8) return false;
+    S.st = std::min(shaped ? 32 : 16, (threads / na) / 8 * 8);
     static size_t budget = gemm_lds_budget();
     auto fits = [&](int st, int sw) { return adjg_lds_bytes<T>(na, S.rows, st, sw, shaped) <= budget; };
-    if (!fits(S.st, S.st) && S.st > 8) S.st = 8;
-    if (!fits(S.st, S.st)) return false;
+    while (S.st > 8 && !fits(S.st, S.st)) S.st -= 8;
+    if (S.st < 8 || !fits(S.st, S.st)) return false;
     // the widest flush window that fits, up to 32 tiles
     S.sw = S.st;
     while (S.sw < 32 * S.st && fits(S.st, S.sw + S.st)) S.sw += S.st;
     lds = adjg_lds_bytes<T>(na, S.rows, S.st, S.sw, shaped);
-    return (S.rows / 16) * (S.st / 8) <= kAdjMfmaWaves * kAdjMaxTiles;
+    return shaped || (S.rows / 16) * (S.st / 8) <= kAdjMfmaWaves * kAdjMaxTiles;
 }
 
 template <typename T>
@@ -945,8 +938,8 @@ static int launch_adjoint_passes(const AdjArgs<T> &Ag, const AdjGemmShape &S, si
         size_t wlds;
         bool wide = false;
         if constexpr (sizeof(T) == 4) {
-            constexpr int wth = adjg_threads<T, true>(), wsp = adjg_split<T, true>();
-            wide = adjg_shape<T>(d, SW, wlds, true, wth, wsp);
+            constexpr int wth = adjg_threads<T, true>();
+            wide = adjg_shape<T>(d, SW, wlds, true, wth);
             if (wide)
                 hipLaunchKernelGGL((k_adj_gemm<T, true, true, BATCHED>), dim3(nwg), dim3(wth), wlds, stream,
                                    Ag, SW);
