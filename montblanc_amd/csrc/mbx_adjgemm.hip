@@ -161,7 +161,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
     // current one's Jones production)
     double *Lsd0 = sacc + S.sw * 10;                                // [2][st][4]: l, m, n - 1, 1 / n
     T *Lsh0 = reinterpret_cast<T *>(Lsd0 + 2 * st * 4);             // [2][st][8]: shape record
-    T *Lst0 = Lsh0 + 2 * st * 8;                                    // [2][st][16]: IQUV P . sqrt(B)
+    T *Lst0 = Lsh0 + 2 * st * 8;                                    // [2][st][16]: IQUV, src_bright, sqrt(B)
     T *Luv = Lst0 + 2 * st * 16;                                    // shaped: (u, v) per antenna
     double *Ltab = reinterpret_cast<double *>(Luv + 2 * NA);        // shaped float64: exp table
 
@@ -227,7 +227,9 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
         const T *St = R.stokes + (uint64_t(s0t + si) * NTT + t) * 4;
         T *o = Lst + 16 * si;
         o[0] = St[0]; o[1] = St[1]; o[2] = St[2]; o[3] = St[3];
-        o[4] = exp_t<T>(double(R.alpha[uint64_t(s0t + si) * NTT + t]) * Lch[4]);
+        const SrcBright<T> sbr =
+            src_bright<T>(St[0], St[1], St[2], St[3], exp_t<T>(double(R.alpha[uint64_t(s0t + si) * NTT + t]) * Lch[4]));
+        o[4] = sbr.rP; o[5] = sbr.s0; o[6] = sbr.it0; o[7] = sbr.is0;
         reinterpret_cast<J4<T> *>(o + 8)[0] = B;
         if (SHAPED) {
             // shape record: gaussian 1, k, k R, el, em, R (k = gauss_scale f);
@@ -279,7 +281,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             else
                 J = item_forward<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[0], an[1], an[2],
                                     an[3], c, Lch[3], Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], Lsd[4 * isi + 2],
-                                    o[0], o[1], o[2], o[3], o[4], ist);
+                                    o[0], o[1], o[2], o[3], SrcBright<T>{o[4], o[5], o[6], o[7]}, ist);
             T *xrec = Xr + isi * XPS + 10 * ia;
             if (SHAPED) {
 #pragma unroll
@@ -522,13 +524,14 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     const T *o = Lst + 16 * si;
                     (void)item_forward<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[0], an[1], an[2],
                                           an[3], c, Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1],
-                                          Lsd[4 * si + 2], o[0], o[1], o[2], o[3], o[4], ist);
+                                          Lsd[4 * si + 2], o[0], o[1], o[2], o[3],
+                                          SrcBright<T>{o[4], o[5], o[6], o[7]}, ist);
                     double g[7];
                     item_backward<T>(ist,
                                      item_chain<T>(R, Luvw + 3 * p, Lpa[2 * p], Lpa[2 * p + 1], an[2], an[3],
                                                    Lch[3], Lch[4], Lsd[4 * si], Lsd[4 * si + 1],
                                                    Lsd[4 * si + 3], ist.okl, ist.okm),
-                                     acc, g);
+                                     acc, o[7], g);
 #pragma unroll
                     for (int x = 0; x < 7; ++x) atomicAdd(sac + si * 10 + x, g[x]);
                 }
@@ -543,7 +546,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                              item_chain<T>(R, Luvw + 3 * ia, Lpa[2 * ia], Lpa[2 * ia + 1], an[2], an[3], Lch[3],
                                            Lch[4], Lsd[4 * isi], Lsd[4 * isi + 1], Lsd[4 * isi + 3], ist.okl,
                                            ist.okm),
-                             Jb[isi * NA + ia], g);
+                             Jb[isi * NA + ia], Lst[16 * isi + 7], g);
 #pragma unroll
             for (int x = 0; x < 7; ++x) atomicAdd(sac + isi * 10 + x, g[x]);
         }
@@ -603,7 +606,7 @@ template <typename T> struct AdjPoint {
     static constexpr int kProducers = 512;
     static constexpr int kThreads = kProducers + 64 * kMaxRb;
     static constexpr int kTaps = sizeof(T) == 4 ? 4 : 2;     // beam taps in flight
-    static constexpr int kParams = 17;                       // per source and slot
+    static constexpr int kParams = 20;                       // per source and slot
     // which role stages the next tile's parameters: in float64 the matrix
     // waves (registers to spare), in float the producers (the matrix
     // waves' W leaves no room)
@@ -676,7 +679,8 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
     auto tile_n = [&](int it) { return min(st, np - it * st); };
 
     // parameters of tile `it` by the first matrix wave: l, m, n - 1, I, Q,
-    // U, V, P = (f / f_ref)^alpha, sqrt(B) (8 reals), 1 / n
+    // U, V, sqrt(P) (P = (f / f_ref)^alpha), sqrt(B) (8 reals), 1 / n, then
+    // s0, 1 / t0, 1 / s0 (src_bright)
     auto stage_params = [&](int it, int si) {
         if (it >= ntiles || si >= tile_n(it)) return;
         const int s = it * st + si;
@@ -686,7 +690,10 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
         double v[C::kParams];
         v[0] = l0; v[1] = m0; v[2] = nm1;
         v[3] = Sk[0]; v[4] = Sk[1]; v[5] = Sk[2]; v[6] = Sk[3];
-        v[7] = exp_t<T>(double(R.alpha[uint64_t(s) * NT + t]) * Lch[4]);
+        const SrcBright<T> sbr = src_bright<T>(Sk[0], Sk[1], Sk[2], Sk[3],
+                                               exp_t<T>(double(R.alpha[uint64_t(s) * NT + t]) * Lch[4]));
+        v[7] = sbr.rP;
+        v[17] = sbr.s0; v[18] = sbr.it0; v[19] = sbr.is0;
 #pragma unroll
         for (int x = 0; x < 4; ++x) { v[8 + 2 * x] = B.v[x].re; v[9 + 2 * x] = B.v[x].im; }
         v[16] = 1.0 / (nm1 + 1.0);
@@ -758,12 +765,15 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
                     (void)item_forward<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2],
                                           an[3], c, Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si],
                                           prm(it, 2)[si], T(prm(it, 3)[si]), T(prm(it, 4)[si]),
-                                          T(prm(it, 5)[si]), T(prm(it, 6)[si]), T(prm(it, 7)[si]), ist);
+                                          T(prm(it, 5)[si]), T(prm(it, 6)[si]),
+                                          SrcBright<T>{T(prm(it, 7)[si]), T(prm(it, 17)[si]), T(prm(it, 18)[si]),
+                                                       T(prm(it, 19)[si])},
+                                          ist);
                     item_backward<T>(ist,
                                      item_chain<T>(R, Luvw + 3 * a, Lpa[2 * a], Lpa[2 * a + 1], an[2], an[3],
                                                    Lch[3], Lch[4], prm(it, 0)[si], prm(it, 1)[si],
                                                    prm(it, 16)[si], ist.okl, ist.okm),
-                                     Jb, gr);
+                                     Jb, T(prm(it, 19)[si]), gr);
                     double *sa = sacc + ((it / S.wt) & 1) * wsrc * 7 + ((it % S.wt) * st + si) * 7;
 #pragma unroll
                     for (int x = 0; x < 7; ++x) atomicAdd(sa + x, gr[x]);

@@ -89,6 +89,22 @@ template <typename T> struct ItemState {
     bool okl, okm;
 };
 
+// Brightness terms of one source at one (time, channel), shared by its
+// items: sqrt(P) (P = (f/f_ref)^alpha), s0 = sqrt(det B0), 1 / t0 with
+// t0 = sqrt(tr B0 + 2 s0) (0 where t0 = 0), 1 / s0 (0 where s0 = 0).
+template <typename T> struct SrcBright { T rP, s0, it0, is0; };
+
+template <typename T>
+__device__ __forceinline__ SrcBright<T> src_bright(T I, T Q, T U, T V, T P) {
+    SrcBright<T> b;
+    b.rP = sqrt(P);
+    b.s0 = sqrt(I * I - Q * Q - U * U - V * V);
+    const T t0 = sqrt(T(2) * I + T(2) * b.s0);
+    b.it0 = t0 != T(0) ? T(1) / t0 : T(0);
+    b.is0 = b.s0 != T(0) ? T(1) / b.s0 : T(0);
+    return b;
+}
+
 // Chain-rule factors of one item from (l, m) to its phase and its beam
 // grid position, and ln(f / f_ref) (inputs as item_forward).
 struct ItemChain {
@@ -111,8 +127,8 @@ __device__ __forceinline__ ItemChain item_chain(const RimeArgs<T> &A, const doub
 }
 
 // Forward of one item with the derivative information its backward needs.
-// Inputs: the source's l, m; its Stokes I, Q, U, V and spectral power
-// P = (f/f_ref)^alpha at this (time, channel); the antenna's uvw (xa),
+// Inputs: the source's l, m; its Stokes I, Q, U, V and brightness terms
+// (src_bright) at this (time, channel); the antenna's uvw (xa),
 // parallactic sin/cos (ps, pc), pointing error (pel, pem) and scaling
 // (al, am); the channel c (beam plane), its frequency f and
 // lnr = log(f/f_ref); nm1 = n - 1 of the source (n = sqrt(1 - l0^2 - m0^2),
@@ -121,8 +137,8 @@ template <typename T>
 __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double *xa, double ps,
                                               double pc, double pel, double pem, double al,
                                               double am, int c, double f, double lnr, double l0,
-                                              double m0, double nm1, T I, T Q, T U, T V, T P,
-                                              ItemState<T> &st) {
+                                              double m0, double nm1, T I, T Q, T U, T V,
+                                              const SrcBright<T> &sb, ItemState<T> &st) {
     const mbx_dims &d = A.d;
     // ---- beam value and its derivatives along the grid fractions
     const double lr = (l0 * pc - m0 * ps + pel) * al;
@@ -139,33 +155,35 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
     const uint32_t o00 = uint32_t(__umul24(uint32_t(__umul24(uint32_t(gl0), MH)) + uint32_t(gm0), cell));
     const uint32_t dl = gl0 < LW1 ? uint32_t(__umul24(MH, cell)) : 0u, dm = gm0 < MH1 ? cell : 0u;
     const char *plane = reinterpret_cast<const char *>(A.beam + uint64_t(c) * d.beam_lw.local_size * MH);
+    // the four taps: cells (g0, g0), (g0 + 1, g0), (g0, g0 + 1), (g0 + 1, g0 + 1)
+    // in (l, m); each of the 12 reals (re, im, |.| of 4 correlations) and its
+    // derivatives along the grid fractions from the differences
+    // a = v10 - v00, c = v01 - v00, e = v11 - v01 - a:
+    //   v = v00 + ld a + md (c + ld e),  dv/dld = a + md e,  dv/dmd = c + ld e
+    const BeamCell<T> e00 = *reinterpret_cast<const BeamCell<T> *>(plane + o00);
+    const BeamCell<T> e10 = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + dl);
+    const BeamCell<T> e01 = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + dm);
+    const BeamCell<T> e11 = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + dl + dm);
+    T Bv[12], dBl[12], dBm[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        const T a = e10.v[i] - e00.v[i], cc = e01.v[i] - e00.v[i];
+        const T ee = (e11.v[i] - e01.v[i]) - a;
+        const T cm = fma(ld, ee, cc);
+        Bv[i] = fma(ld, a, fma(md, cm, e00.v[i]));
+        dBl[i] = fma(md, ee, a);
+        dBm[i] = cm;
+    }
     cx<T> S[4], dSl[4], dSm[4];
     T Am[4], dAl[4], dAm[4];
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
-        S[x] = dSl[x] = dSm[x] = {T(0), T(0)};
-        Am[x] = dAl[x] = dAm[x] = T(0);
-    }
-    // bilinear taps of the channel's plane, and their weights' derivatives
-    // along the l and m grid fractions
-#pragma unroll 1
-    for (int tap = 0; tap < 4; ++tap) {
-        const bool hl = tap & 1, hm = (tap >> 1) & 1;
-        const T wl = hl ? ld : T(1) - ld, wm = hm ? md : T(1) - md;
-        const T w = wl * wm;
-        const T wdl = hl ? wm : -wm;
-        const T wdm = hm ? wl : -wl;
-        const BeamCell<T> e = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + (hl ? dl : 0u) + (hm ? dm : 0u));
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            const T vre = e.v[3 * x], vim = e.v[3 * x + 1], vamp = e.v[3 * x + 2];
-            S[x].re = fma(w, vre, S[x].re);     S[x].im = fma(w, vim, S[x].im);
-            dSl[x].re = fma(wdl, vre, dSl[x].re); dSl[x].im = fma(wdl, vim, dSl[x].im);
-            dSm[x].re = fma(wdm, vre, dSm[x].re); dSm[x].im = fma(wdm, vim, dSm[x].im);
-            Am[x] = fma(w, vamp, Am[x]);
-            dAl[x] = fma(wdl, vamp, dAl[x]);
-            dAm[x] = fma(wdm, vamp, dAm[x]);
-        }
+        S[x] = {Bv[3 * x], Bv[3 * x + 1]};
+        dSl[x] = {dBl[3 * x], dBl[3 * x + 1]};
+        dSm[x] = {dBm[3 * x], dBm[3 * x + 1]};
+        Am[x] = Bv[3 * x + 2];
+        dAl[x] = dBl[3 * x + 2];
+        dAm[x] = dBm[3 * x + 2];
     }
     J4<T> E, dEl, dEm;
 #pragma unroll
@@ -192,13 +210,10 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
     T sn, cs;
     PhaseCS<T>::eval(phi, &sn, &cs);
     const cx<T> K = {cs, sn};
-    const T rP = sqrt(P);
-    const T s0 = sqrt(I * I - Q * Q - U * U - V * V);
-    const T t0 = sqrt(T(2) * I + T(2) * s0);
+    const T rP = sb.rP, s0 = sb.s0, it0 = sb.it0;
     J4<T> B0s;                            // sqrt(B0), B0 = unscaled brightness
     B0s.v[0] = {I + Q + s0, T(0)}; B0s.v[1] = {U, V}; B0s.v[2] = {U, -V}; B0s.v[3] = {I - Q + s0, T(0)};
-    const T it0 = t0 != T(0) ? T(1) / t0 : T(0);
-    if (t0 != T(0)) B0s = jscale(B0s, it0);
+    if (it0 != T(0)) B0s = jscale(B0s, it0);
     const J4<T> sB = jscale(B0s, rP);      // sqrt(B) = sqrt(P) sqrt(B0)
     J4<T> X;
 #pragma unroll
@@ -210,11 +225,11 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
 }
 
 // Backward of one item: d chi^2 / d(l, m, I, Q, U, V, alpha) into g from
-// the Jones adjoint Jb, the item's forward state and its chain-rule
-// factors (item_chain).
+// the Jones adjoint Jb, the item's forward state, its chain-rule factors
+// (item_chain) and the source's 1 / s0 (src_bright).
 template <typename T>
 __device__ __forceinline__ void item_backward(const ItemState<T> &st, const ItemChain &ch,
-                                              const J4<T> &Jb, double *g /* [7] */) {
+                                              const J4<T> &Jb, T is0, double *g /* [7] */) {
     const J4<T> &E = st.E, &dEl = st.dEl, &dEm = st.dEm;
     const cx<T> K = st.K;
     const T rP = st.rP, s0 = st.s0, it0 = st.it0, I = st.I, Q = st.Q, U = st.U, V = st.V;
@@ -251,7 +266,6 @@ __device__ __forceinline__ void item_backward(const ItemState<T> &st, const Item
         const T ZB = jdot_re(Zb, B0s);
         const T rdB[4] = {Zb.v[0].re + Zb.v[3].re, Zb.v[0].re - Zb.v[3].re,
                           Zb.v[1].re + Zb.v[2].re, Zb.v[1].im - Zb.v[2].im};
-        const T is0 = T(1) / s0;
         const T dsx[4] = {I * is0, -Q * is0, -U * is0, -V * is0};
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
@@ -276,9 +290,10 @@ __device__ __forceinline__ void antenna_backward_at(const RimeArgs<T> &A, const 
                                                     double *g /* [7] */) {
     ItemState<T> st;
     const double n = sqrt(1.0 - l0 * l0 - m0 * m0);
-    (void)item_forward<T>(A, xa, ps, pc, pel, pem, al, am, c, f, lnr, l0, m0, n - 1.0, I, Q, U, V, P, st);
+    const SrcBright<T> sb = src_bright<T>(I, Q, U, V, P);
+    (void)item_forward<T>(A, xa, ps, pc, pel, pem, al, am, c, f, lnr, l0, m0, n - 1.0, I, Q, U, V, sb, st);
     item_backward<T>(st, item_chain<T>(A, xa, ps, pc, al, am, f, lnr, l0, m0, 1.0 / n, st.okl, st.okm), Jb,
-                     g);
+                     sb.is0, g);
 }
 
 // The same item reading its inputs from global memory and the tile's
