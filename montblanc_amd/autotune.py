@@ -108,6 +108,10 @@ def _probe_dims(arrays, counts, ntime_probe):
     return d
 
 
+# threads per workgroup of the persistent batched kernel (k_batched)
+BATCH_THREADS = 512
+
+
 def batched_tuning(batch, gradients=False, ntime_probe=2):
     """Launch shape (or strategy) of the batched evaluation for this batch.
 
@@ -128,8 +132,13 @@ def batched_tuning(batch, gradients=False, ntime_probe=2):
     props = torch.cuda.get_device_properties(batch.device)
     cus = props.multi_processor_count
     cands = []
+    nbl = a['antenna1'].shape[1]
     for wg in (cus, 2 * cus, 4 * cus):
         for cpi in (1, 2):
+            # more than four (baseline, channel) items per thread of the
+            # 512-thread persistent kernel spill its accumulators to scratch
+            if nbl * cpi > 4 * BATCH_THREADS:
+                continue
             for st in (0, 8):
                 cands.append({'workgroups': wg, 'channels_per_item': cpi, 'sources_per_tile': st})
     # the matrix-core passes, each one launch over groups of samples (float,
