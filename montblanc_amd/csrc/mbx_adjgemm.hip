@@ -835,7 +835,10 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
 #pragma unroll
                 for (int cb = 0; cb < C::kMaxCb; ++cb) {
                     if (cb >= st / 8) break;
-                    acc_t ore = acc_t{T(0), T(0), T(0), T(0)}, oim = ore;
+                    // three real products per complex one (Gauss):
+                    // P1 = Re W Re X, P2 = Im W Im X, P3 = (Re W + Im W)(Re X + Im X);
+                    // Re Jb = P1 - P2, Im Jb = P3 - P1 - P2
+                    acc_t p1 = acc_t{T(0), T(0), T(0), T(0)}, p2 = p1, p3 = p1;
                     const int xn = 16 * cb + i16;
 #pragma unroll
                     for (int kb = 0; kb < C::kMaxRb; ++kb) {
@@ -846,11 +849,16 @@ k_adj_point(AdjArgs<T> A, AdjPointShape S) {
                         for (int m = 0; m < 4; ++m) { x4[m] = xr[o + m]; y4[m] = xi[o + m]; }
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
-                            ore = Mfma<T>::mma(Wr[kb][m], x4[m], ore);
-                            ore = Mfma<T>::mma(Wi[kb][m], -y4[m], ore);
-                            oim = Mfma<T>::mma(Wi[kb][m], x4[m], oim);
-                            oim = Mfma<T>::mma(Wr[kb][m], y4[m], oim);
+                            p1 = Mfma<T>::mma(Wr[kb][m], x4[m], p1);
+                            p2 = Mfma<T>::mma(Wi[kb][m], y4[m], p2);
+                            p3 = Mfma<T>::mma(Wr[kb][m] + Wi[kb][m], x4[m] + y4[m], p3);
                         }
+                    }
+                    acc_t ore, oim;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        ore[r] = p1[r] - p2[r];
+                        oim[r] = p3[r] - p1[r] - p2[r];
                     }
                     const int j = xn / st, si = xn % st;
 #pragma unroll
