@@ -327,32 +327,38 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
             typedef typename Mfma<T>::acc_t acc_t;
             const int nR = rows / 16, ncb = (2 * st) / 16, nout = nR * ncb;
             const int i16 = lane & 15, g = lane >> 4;
+            // three real products per complex one, as in k_adj_point:
+            // Re Jb = P1 - P2, Im Jb = P3 - P1 - P2
             acc_t ore[kAdjMaxTiles], oim[kAdjMaxTiles];
             if (wave < kAdjMfmaWaves) {
 #pragma unroll
                 for (int j = 0; j < kAdjMaxTiles; ++j) {
-                    ore[j] = acc_t{T(0), T(0), T(0), T(0)};
-                    oim[j] = acc_t{T(0), T(0), T(0), T(0)};
+                    acc_t p1 = acc_t{T(0), T(0), T(0), T(0)}, p2 = p1, p3 = p1;
                     const int o = wave + kAdjMfmaWaves * j;
-                    if (o >= nout) continue;
-                    const int Rb = o / ncb, Cb = o % ncb;
-                    const int wrow = 16 * Rb + Mfma<T>::arow(i16), xn = 16 * Cb + i16;
-                    for (int kk = 0; kk < rows; kk += 16) {
-                        const T *wr = Wr + wi_(wrow, kk + 4 * g), *wi = Wi + wi_(wrow, kk + 4 * g);
-                        const T *xr = Xr + xi_(xn, kk + 4 * g), *xi = Xi + xi_(xn, kk + 4 * g);
-                        T a4[4], b4[4], x4[4], y4[4];
+                    if (o < nout) {
+                        const int Rb = o / ncb, Cb = o % ncb;
+                        const int wrow = 16 * Rb + Mfma<T>::arow(i16), xn = 16 * Cb + i16;
+                        for (int kk = 0; kk < rows; kk += 16) {
+                            const T *wr = Wr + wi_(wrow, kk + 4 * g), *wi = Wi + wi_(wrow, kk + 4 * g);
+                            const T *xr = Xr + xi_(xn, kk + 4 * g), *xi = Xi + xi_(xn, kk + 4 * g);
+                            T a4[4], b4[4], x4[4], y4[4];
 #pragma unroll
-                        for (int m = 0; m < 4; ++m) {
-                            a4[m] = wr[m]; b4[m] = wi[m];
-                            x4[m] = xr[m]; y4[m] = xi[m];
-                        }
+                            for (int m = 0; m < 4; ++m) {
+                                a4[m] = wr[m]; b4[m] = wi[m];
+                                x4[m] = xr[m]; y4[m] = xi[m];
+                            }
 #pragma unroll
-                        for (int m = 0; m < 4; ++m) {
-                            ore[j] = Mfma<T>::mma(a4[m], x4[m], ore[j]);
-                            ore[j] = Mfma<T>::mma(-b4[m], y4[m], ore[j]);
-                            oim[j] = Mfma<T>::mma(b4[m], x4[m], oim[j]);
-                            oim[j] = Mfma<T>::mma(a4[m], y4[m], oim[j]);
+                            for (int m = 0; m < 4; ++m) {
+                                p1 = Mfma<T>::mma(a4[m], x4[m], p1);
+                                p2 = Mfma<T>::mma(b4[m], y4[m], p2);
+                                p3 = Mfma<T>::mma(a4[m] + b4[m], x4[m] + y4[m], p3);
+                            }
                         }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        ore[j][r] = p1[r] - p2[r];
+                        oim[j][r] = p3[r] - p1[r] - p2[r];
                     }
                 }
             }
