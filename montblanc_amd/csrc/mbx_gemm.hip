@@ -207,10 +207,16 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
         }
         const int ntp_mine = max(0, min(G.tpw, G.ntp - wave * G.tpw));
         acc_t are[kGemmMaxTp], aim[kGemmMaxTp];
+        // float64 point pass: three real products per complex one (Gauss),
+        // are = P1 (Re Re), aim = P2 (Im Im), a3 = P3 ((Re + Im)(Re - Im)^T),
+        // combined before the epilogue: Re V = P1 + P2, Im V = P3 - P1 + P2
+        constexpr bool kGauss3 = sizeof(T) == 8 && !SHAPED;
+        acc_t a3[kGauss3 ? kGemmMaxTp : 1];
 #pragma unroll
         for (int j = 0; j < kGemmMaxTp; ++j) {
             are[j] = acc_t{T(0), T(0), T(0), T(0)};
             aim[j] = acc_t{T(0), T(0), T(0), T(0)};
+            if constexpr (kGauss3) a3[j] = acc_t{T(0), T(0), T(0), T(0)};
         }
         // Im part of a shaped tile's product: lane group g supplies
         // component (g + 2) mod 4 of A' = [Im X | -Re X]
@@ -229,11 +235,20 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                         if (j < ntp_mine) {
                             const XRow<T> a = Xs[16 * tpR[j] + Mfma<T>::arow(i16)];
                             const XRow<T> b = Xs[16 * tpC[j] + i16];
-                            const T ai[4] = {a.v[2], a.v[3], -a.v[0], -a.v[1]};
+                            if constexpr (kGauss3) {
 #pragma unroll
-                            for (int m = 0; m < 4; ++m) {
-                                are[j] = Mfma<T>::mma(a.v[m], b.v[m], are[j]);
-                                aim[j] = Mfma<T>::mma(ai[m], b.v[m], aim[j]);
+                                for (int m = 0; m < 2; ++m) {
+                                    are[j] = Mfma<T>::mma(a.v[m], b.v[m], are[j]);
+                                    aim[j] = Mfma<T>::mma(a.v[m + 2], b.v[m + 2], aim[j]);
+                                    a3[j] = Mfma<T>::mma(a.v[m] + a.v[m + 2], b.v[m] - b.v[m + 2], a3[j]);
+                                }
+                            } else {
+                                const T ai[4] = {a.v[2], a.v[3], -a.v[0], -a.v[1]};
+#pragma unroll
+                                for (int m = 0; m < 4; ++m) {
+                                    are[j] = Mfma<T>::mma(a.v[m], b.v[m], are[j]);
+                                    aim[j] = Mfma<T>::mma(ai[m], b.v[m], aim[j]);
+                                }
                             }
                             // float64 operands take twice the registers: one
                             // tile pair's loads in flight at a time
@@ -298,6 +313,16 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                 }
             }
             __syncthreads();
+        }
+        if constexpr (kGauss3) {
+#pragma unroll
+            for (int j = 0; j < kGemmMaxTp; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const T p1 = are[j][r], p2 = aim[j][r];
+                    are[j][r] = p1 + p2;
+                    aim[j][r] = a3[j][r] - p1 + p2;
+                }
         }
         // epilogue: scatter the upper-triangle elements to their baselines
         const uint64_t NC = d.nchan.local_size, NBL = d.nbl.local_size;
