@@ -67,7 +67,7 @@ static inline size_t adjg_lds_bytes(int na, int rows, int st, int sw, bool shape
            + 2 * size_t(st) * 4 * sizeof(double)                        // l, m, n - 1, 1 / n (2 tiles)
            + 2 * size_t(st) * 24 * sizeof(T)                            // shape, I Q U V P, sqrt(B)
            + (shaped ? 2 * size_t(na) : 0) * sizeof(T)                  // u, v
-           + (shaped && sizeof(T) == 8 ? kExpTab * sizeof(double) : 0); // exp_nonpos table
+           + (shaped && sizeof(T) == 8 ? kExpTab * sizeof(double) : 0); // exp_neg table
 }
 
 // Gaussian shape coordinates are kept scaled by sqrt(log2 e) in float, so
@@ -448,7 +448,7 @@ k_adj_gemm(AdjArgs<T> A, AdjGemmShape S) {
                     const T r2 = fma(dx, dx, dy * dy);
                     T f, id = T(1);
                     if constexpr (GAUSS) {
-                        if constexpr (sizeof(T) == 8) f = exp_nonpos(-r2, Ltab);
+                        if constexpr (sizeof(T) == 8) f = exp_neg(r2, Ltab);
                         else f = gauss_of_scaled(r2);
                     } else {
                         f = sersic_factor(r2, id);

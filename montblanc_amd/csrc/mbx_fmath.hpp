@@ -7,8 +7,8 @@
 
 namespace mbx {
 
-// Entries of the exp_nonpos table: 2^(j / 64), j = 0..63.
-constexpr int kExpTab = 64;
+// Entries of the exp_neg table: 2^(j / 256), j = 0..255.
+constexpr int kExpTab = 256;
 
 // Fill the table (LDS) with threads [0, nth) of a workgroup; the caller
 // synchronises before use.
@@ -16,29 +16,29 @@ __device__ __forceinline__ void exp_table_fill(double *tab, int tid, int nth) {
     for (int j = tid; j < kExpTab; j += nth) tab[j] = exp2(double(j) / kExpTab);
 }
 
-// exp(x), x <= 0, by Tang's table-driven method (P. T. P. Tang, "Table-
+// exp(-x), x >= 0, by Tang's table-driven method (P. T. P. Tang, "Table-
 // driven implementation of the exponential function in IEEE floating-point
-// arithmetic", ACM TOMS 15(2), 1989): x = (64 m + j) ln2 / 64 + r with
-// |r| <= ln2 / 128, the reduction in two steps (Cody and Waite) with ln2
+// arithmetic", ACM TOMS 15(2), 1989): -x = (256 m + j) ln2 / 256 + r with
+// |r| <= ln2 / 512, the reduction in two steps (Cody and Waite) with ln2
 // split into a 32-bit head, whose products with k are exact, and a tail;
-// exp(r) by its Taylor polynomial to degree 5 (truncation below 4e-17 at
-// |r| = ln2 / 128); 2^(j / 64) from the table; 2^m by ldexp. Arguments
-// below -745 give 0 (as exp does, past the smallest subnormal).
-__device__ __forceinline__ double exp_nonpos(double x, const double *tab) {
-    constexpr double kInvStep = 92.332482616893656877;     // 64 / ln 2
-    constexpr double kStepHi = 0x1.62e42fee00000p-1 / 64;  // ln 2 / 64, head
-    constexpr double kStepLo = 0x1.a39ef35793c76p-33 / 64; // ln 2 / 64, tail
-    x = fmax(x, -745.5);
-    const double k = rint(x * kInvStep);
-    double r = fma(-k, kStepHi, x);
+// exp(r) by its Taylor polynomial to degree 4 (truncation below 4e-17 at
+// |r| = ln2 / 512); 2^(j / 256) from the table; 2^m by ldexp. Arguments
+// above 745 give 0 (as exp does, past the smallest subnormal). x comes
+// from arithmetic (never a signalling NaN), so the clamp is one v_min.
+__device__ __forceinline__ double exp_neg(double x, const double *tab) {
+    constexpr double kInvStep = 369.32993046757462751;      // 256 / ln 2
+    constexpr double kStepHi = 0x1.62e42fee00000p-1 / 256; // ln 2 / 256, head
+    constexpr double kStepLo = 0x1.a39ef35793c76p-33 / 256; // ln 2 / 256, tail
+    x = fmin(x, 745.5);
+    const double k = rint(x * -kInvStep);
+    double r = fma(-k, kStepHi, -x);
     r = fma(-k, kStepLo, r);
-    double p = fma(r, 1.0 / 120, 1.0 / 24);
-    p = fma(p, r, 1.0 / 6);
+    double p = fma(r, 1.0 / 24, 1.0 / 6);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
     const int ki = int(k);
-    return ldexp(p * tab[ki & (kExpTab - 1)], ki >> 6);   // ki = 64 m + j, m = floor(ki / 64)
+    return ldexp(p * tab[ki & (kExpTab - 1)], ki >> 8);   // ki = 256 m + j, m = floor(ki / 256)
 }
 
 // 1 / sqrt(d) for d >= 1: the hardware reciprocal square root refined by

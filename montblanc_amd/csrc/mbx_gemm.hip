@@ -71,7 +71,7 @@ static __host__ __device__ inline size_t gemm_x_offset(int na) {
 }
 
 // Then the two source-record buffers, and in the shaped pass the shape
-// coordinates and kinds of both tiles (and in float64 the exp_nonpos table).
+// coordinates and kinds of both tiles (and in float64 the exp_neg table).
 template <typename T, bool SHAPED>
 static inline size_t gemm_lds_bytes(int na, int st) {
     return gemm_x_offset(na) + 2 * size_t(st) * gemm_rows(na) * sizeof(XRow<T>) +
@@ -104,6 +104,18 @@ template <> __device__ __forceinline__ double shape_f<double>(bool gauss, double
     const double den = 1.0 + r2;
     return 1.0 / (den * sqrt(den));
 }
+
+// x with its sign bit flipped where m = 0x80000000 (m = 0: x).
+__device__ __forceinline__ float flip_sign(float x, unsigned m) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) ^ m);
+}
+__device__ __forceinline__ double flip_sign(double x, unsigned m) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^
+                                          (static_cast<unsigned long long>(m) << 32));
+}
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { typedef float2 type; };
+template <> struct Vec2<double> { typedef double2 type; };
 
 // The value of lane l ^ 1 (DPP quad permutation 1, 0, 3, 2; no LDS).
 __device__ __forceinline__ float swap_lane_pairs(float v) {
@@ -221,6 +233,30 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
         // Im part of a shaped tile's product: lane group g supplies
         // component (g + 2) mod 4 of A' = [Im X | -Re X]
         const acc_t zero = acc_t{T(0), T(0), T(0), T(0)};
+        // shaped pass, per tile pair j: byte offsets of this lane's A and B
+        // reals in a source's rows (A: row 16 R + arow(i16), B: row 16 C +
+        // i16, real g of each) and of the shape coordinates of its column
+        // antenna q and row antenna pm. The lane's rows are antennas p0 =
+        // 8 R + 2 g and p0 + 1 (rows past 2 na: any finite factor,
+        // discarded); its neighbour (lane ^ 1, the other column of antenna
+        // q) needs the same two factors: each evaluates one (pm = p0 + the
+        // lane's parity) and takes the other across.
+        int offA[SHAPED ? kGemmMaxTp : 1], offB[SHAPED ? kGemmMaxTp : 1];
+        int offQ[SHAPED ? kGemmMaxTp : 1], offP[SHAPED ? kGemmMaxTp : 1];
+        const int dA2 = (((g + 2) & 3) - g) * int(sizeof(T));
+        const unsigned sgn = g < 2 ? 0u : 0x80000000u;
+        const bool odd = (lane & 1) != 0;
+        if constexpr (SHAPED) {
+#pragma unroll
+            for (int j = 0; j < kGemmMaxTp; ++j) {
+                const int ra = 16 * tpR[j] + Mfma<T>::arow(i16), cb = 16 * tpC[j] + i16;
+                offA[j] = (ra * 4 + g) * int(sizeof(T));
+                offB[j] = (cb * 4 + g) * int(sizeof(T));
+                offQ[j] = min(cb >> 1, NA - 1) * 2 * int(sizeof(T));
+                offP[j] = min(8 * tpR[j] + 2 * g + (lane & 1), NA - 1) * 2 * int(sizeof(T));
+            }
+        }
+
         __syncthreads();                      // tile 0 produced
         for (int it = 0; it < ntiles; ++it) {
             const int buf = it & 1;
@@ -257,51 +293,43 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                     }
                 }
             } else {
-                // one source per MFMA (k = the four reals of its Jones
-                // row), each product scaled per element by f_pq. Same
-                // source-major rows as the point pass, so a lane's read of
-                // one real of a row is conflict-free; shape coordinates
-                // (x, y) interleaved per (source, antenna).
-                const T *Xg = Xbase + buf * xstride;
-                const T *XY = XYbase + buf * xystride;
+                // one source per MFMA (k = the four reals of its Jones row), each
+                // product scaled per element by f_pq. Same source-major rows as
+                // the point pass, so a lane's read of one real of a row is
+                // conflict-free; shape coordinates (x, y) interleaved per
+                // (source, antenna). The lane's byte offsets within a source's
+                // rows and coordinates are fixed per tile pair (set up before
+                // the tile loop); per source only the source's base is added.
+                const char *Xg = reinterpret_cast<const char *>(Xbase + buf * xstride);
+                const char *XY = reinterpret_cast<const char *>(XYbase + buf * xystride);
                 const int *kind = Kbase + buf * st;
                 for (int s = 0; s < nst; ++s) {
-                    // re-derived per source so the per-tile-pair addresses
-                    // are not all hoisted out of the loop into registers
-                    int lv = lane;
-                    __asm__ volatile("" : "+v"(lv));
-                    const int i16v = lv & 15, gv = lv >> 4;
-                    const int g2 = (gv + 2) & 3;
-                    const T sg = gv < 2 ? T(1) : T(-1);
+                    const char *Xs = Xg + s * rows * 4 * int(sizeof(T));
+                    const char *XYs = XY + s * NA * 2 * int(sizeof(T));
+                    // the kind is the same for the whole wave (a uniform branch)
                     const bool gauss = kind[s] == 1;
-                    const T *Xs = Xg + s * rows * 4;
-                    const T *XYs = XY + s * NA * 2;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
-                            const int ra = 16 * tpR[j] + Mfma<T>::arow(i16v), cb = 16 * tpC[j] + i16v;
-                            const T bv = Xs[cb * 4 + gv];
-                            const acc_t P = Mfma<T>::mma(Xs[ra * 4 + gv], bv, zero);
-                            const acc_t Q = Mfma<T>::mma(sg * Xs[ra * 4 + g2], bv, zero);
-                            const int q = min(cb >> 1, NA - 1);
-                            const T xq = XYs[2 * q], yq = XYs[2 * q + 1];
-                            // this lane's rows are antennas p0 = 8 R + 2 g and p0 + 1
-                            // (rows past 2 na: any finite factor, discarded). Its
-                            // neighbour (lane ^ 1, the other column of antenna q)
-                            // needs the same two factors: each evaluates one and
-                            // takes the other across.
-                            const int p0 = 8 * tpR[j] + 2 * gv;
-                            const int pm = min(p0 + (lv & 1), NA - 1);
+                            const T bv = *reinterpret_cast<const T *>(Xs + offB[j]);
+                            const acc_t P =
+                                Mfma<T>::mma(*reinterpret_cast<const T *>(Xs + offA[j]), bv, zero);
+                            const T a2 = *reinterpret_cast<const T *>(Xs + offA[j] + dA2);
+                            const acc_t Q = Mfma<T>::mma(flip_sign(a2, sgn), bv, zero);
+                            const typename Vec2<T>::type cq =
+                                *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offQ[j]);
+                            const typename Vec2<T>::type cp =
+                                *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offP[j]);
+                            const T dx = cq.x - cp.x, dy = cq.y - cp.y;
                             T fm;
                             if constexpr (sizeof(T) == 8) {
-                                const T dx = xq - XYs[2 * pm], dy = yq - XYs[2 * pm + 1];
                                 const T r2 = fma(dx, dx, dy * dy);
-                                fm = gauss ? exp_nonpos(-r2, Ltab) : rpow32(1.0 + r2);
+                                fm = gauss ? exp_neg(r2, Ltab) : rpow32(1.0 + r2);
                             } else {
-                                fm = shape_f<T>(gauss, xq - XYs[2 * pm], yq - XYs[2 * pm + 1]);
+                                fm = shape_f<T>(gauss, dx, dy);
                             }
                             const T fo = swap_lane_pairs(fm);
-                            const T f0 = (lv & 1) ? fo : fm, f1 = (lv & 1) ? fm : fo;
+                            const T f0 = odd ? fo : fm, f1 = odd ? fm : fo;
                             are[j][0] = fma(f0, P[0], are[j][0]); aim[j][0] = fma(f0, Q[0], aim[j][0]);
                             are[j][1] = fma(f0, P[1], are[j][1]); aim[j][1] = fma(f0, Q[1], aim[j][1]);
                             are[j][2] = fma(f1, P[2], are[j][2]); aim[j][2] = fma(f1, Q[2], aim[j][2]);
