@@ -23,13 +23,17 @@ __device__ __forceinline__ void exp_table_fill(double *tab, int tid, int nth) {
 // split into a 32-bit head, whose products with k are exact, and a tail;
 // exp(r) by its Taylor polynomial to degree 4 (truncation below 4e-17 at
 // |r| = ln2 / 512); 2^(j / 256) from the table; 2^m by ldexp. Arguments
-// above 745 give 0 (as exp does, past the smallest subnormal). x comes
-// from arithmetic (never a signalling NaN), so the clamp is one v_min.
+// above 745.5 give 0 (as exp does, past the smallest subnormal).
 __device__ __forceinline__ double exp_neg(double x, const double *tab) {
     constexpr double kInvStep = 369.32993046757462751;      // 256 / ln 2
     constexpr double kStepHi = 0x1.62e42fee00000p-1 / 256; // ln 2 / 256, head
     constexpr double kStepLo = 0x1.a39ef35793c76p-33 / 256; // ln 2 / 256, tail
-    x = fmin(x, 745.5);
+    // x = min(x, 745.5) on the high word (signed: monotonic for x >= +0;
+    // +inf and NaN clamp too); the low word is kept, so a clamped x lies in
+    // [745.5, 745.5005), where the result is 0 as well
+    const long long xb = __builtin_bit_cast(long long, x);
+    const int xhi = min(int(xb >> 32), 0x40874C00);   // high word of 745.5
+    x = __builtin_bit_cast(double, (static_cast<long long>(xhi) << 32) | static_cast<unsigned int>(xb));
     const double k = rint(x * -kInvStep);
     double r = fma(-k, kStepHi, -x);
     r = fma(-k, kStepLo, r);

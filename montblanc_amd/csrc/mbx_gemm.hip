@@ -117,14 +117,15 @@ template <typename T> struct Vec2;
 template <> struct Vec2<float> { typedef float2 type; };
 template <> struct Vec2<double> { typedef double2 type; };
 
-// The value of lane l ^ 1 (DPP quad permutation 1, 0, 3, 2; no LDS).
-__device__ __forceinline__ float swap_lane_pairs(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+// The value of lane 4 (l / 4) + perm[l % 4] (DPP quad permutation, the
+// four 2-bit lane selects packed in CTRL; no LDS).
+template <int CTRL> __device__ __forceinline__ float quad_perm(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
 }
-__device__ __forceinline__ double swap_lane_pairs(double v) {
+template <int CTRL> __device__ __forceinline__ double quad_perm(double v) {
     const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0xB1, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0xB1, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), CTRL, 0xF, 0xF, false);
     return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
@@ -245,7 +246,6 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
         int offQ[SHAPED ? kGemmMaxTp : 1], offP[SHAPED ? kGemmMaxTp : 1];
         const int dA2 = (((g + 2) & 3) - g) * int(sizeof(T));
         const unsigned sgn = g < 2 ? 0u : 0x80000000u;
-        const bool odd = (lane & 1) != 0;
         if constexpr (SHAPED) {
 #pragma unroll
             for (int j = 0; j < kGemmMaxTp; ++j) {
@@ -328,8 +328,8 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                             } else {
                                 fm = shape_f<T>(gauss, dx, dy);
                             }
-                            const T fo = swap_lane_pairs(fm);
-                            const T f0 = odd ? fo : fm, f1 = odd ? fm : fo;
+                            // rows of p0 from the even lane, of p0 + 1 from the odd one
+                            const T f0 = quad_perm<0xA0>(fm), f1 = quad_perm<0xF5>(fm);
                             are[j][0] = fma(f0, P[0], are[j][0]); aim[j][0] = fma(f0, Q[0], aim[j][0]);
                             are[j][1] = fma(f0, P[1], are[j][1]); aim[j][1] = fma(f0, Q[1], aim[j][1]);
                             are[j][2] = fma(f1, P[2], are[j][2]); aim[j][2] = fma(f1, Q[2], aim[j][2]);
