@@ -304,8 +304,14 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                 const char *XY = reinterpret_cast<const char *>(XYbase + buf * xystride);
                 const int *kind = Kbase + buf * st;
                 for (int s = 0; s < nst; ++s) {
-                    const char *Xs = Xg + s * rows * 4 * int(sizeof(T));
-                    const char *XYs = XY + s * NA * 2 * int(sizeof(T));
+                    // the source's byte offsets, opaque to the compiler: it
+                    // would otherwise step every per-tile-pair address of
+                    // the loop as its own induction variable (an addition
+                    // per address per source on top of the one per read)
+                    int xo = s * rows * 4 * int(sizeof(T)), yo = s * NA * 2 * int(sizeof(T));
+                    __asm__ volatile("" : "+s"(xo), "+s"(yo));
+                    const char *Xs = Xg + xo;
+                    const char *XYs = XY + yo;
                     // the kind is the same for the whole wave (a uniform branch)
                     const bool gauss = kind[s] == 1;
 #pragma unroll
