@@ -246,6 +246,16 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
         int offQ[SHAPED ? kGemmMaxTp : 1], offP[SHAPED ? kGemmMaxTp : 1];
         const int dA2 = (((g + 2) & 3) - g) * int(sizeof(T));
         const unsigned sgn = g < 2 ? 0u : 0x80000000u;
+        // point pass, per tile pair j: byte offsets of this lane's A and B
+        // rows (source g of a group of four)
+        int offRa[SHAPED ? 1 : kGemmMaxTp], offCb[SHAPED ? 1 : kGemmMaxTp];
+        if constexpr (!SHAPED) {
+#pragma unroll
+            for (int j = 0; j < kGemmMaxTp; ++j) {
+                offRa[j] = (g * rows + 16 * tpR[j] + Mfma<T>::arow(i16)) * int(sizeof(XRow<T>));
+                offCb[j] = (g * rows + 16 * tpC[j] + i16) * int(sizeof(XRow<T>));
+            }
+        }
         if constexpr (SHAPED) {
 #pragma unroll
             for (int j = 0; j < kGemmMaxTp; ++j) {
@@ -263,14 +273,19 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
             int s0t, nst;
             tile_range(it, s0t, nst);
             if (!SHAPED) {
-                const XRow<T> *Xb = reinterpret_cast<const XRow<T> *>(Xbase + buf * xstride);
+                const char *Xb = reinterpret_cast<const char *>(Xbase + buf * xstride);
                 for (int s0 = 0; s0 < nst; s0 += 4) {
-                    const XRow<T> *Xs = Xb + (s0 + g) * rows;
+                    // the group's byte offset, opaque to the compiler (else
+                    // each tile pair's two addresses become induction
+                    // variables of their own, stepped per group)
+                    int xo = s0 * rows * int(sizeof(XRow<T>));
+                    __asm__ volatile("" : "+s"(xo));
+                    const char *Xs = Xb + xo;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
-                            const XRow<T> a = Xs[16 * tpR[j] + Mfma<T>::arow(i16)];
-                            const XRow<T> b = Xs[16 * tpC[j] + i16];
+                            const XRow<T> a = *reinterpret_cast<const XRow<T> *>(Xs + offRa[j]);
+                            const XRow<T> b = *reinterpret_cast<const XRow<T> *>(Xs + offCb[j]);
                             if constexpr (kGauss3) {
 #pragma unroll
                                 for (int m = 0; m < 2; ++m) {
