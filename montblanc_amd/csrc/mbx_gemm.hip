@@ -50,14 +50,20 @@ template <typename T> struct alignas(4 * sizeof(T)) XRow { T v[4]; };
 // workgroup -- 12 in float (1024 threads, 128 VGPRs each), 4 in float64
 // (512 threads, 256 VGPRs: the consumers hold 144 of accumulators).
 // Shaped pass, whose consumers also evaluate a shape factor per element:
-// eight consumer waves with up to five tile pairs, 8 producers in float
-// (1024 threads), 4 in float64 (768 threads, 168 VGPRs).
+// in float eight consumer waves with up to five tile pairs and 8 producers;
+// in float64 twelve consumer waves with up to three (three per SIMD, so
+// that the latency of each one's factor arithmetic hides behind the
+// others') and 4 producers; 1024 threads, 128 VGPRs, in both.
 template <typename T, bool SHAPED> constexpr int gemm_threads() {
-    return sizeof(T) == 4 ? 1024 : (SHAPED ? 768 : 512);
+    return sizeof(T) == 4 || SHAPED ? 1024 : 512;
 }
-template <bool SHAPED> constexpr int gemm_consumers() { return SHAPED ? 512 : 256; }
-template <bool SHAPED> constexpr int gemm_max_tp() { return SHAPED ? 5 : 9; }
-constexpr int kGemmMaxRowBlocks = 8;  // 4 * 9 and 8 * 5 >= 8 * 9 / 2: na <= 64
+template <typename T, bool SHAPED> constexpr int gemm_consumers() {
+    return SHAPED ? (sizeof(T) == 8 ? 768 : 512) : 256;
+}
+template <typename T, bool SHAPED> constexpr int gemm_max_tp() {
+    return SHAPED ? (sizeof(T) == 8 ? 3 : 5) : 9;
+}
+constexpr int kGemmMaxRowBlocks = 8;  // 4 * 9, 8 * 5 and 12 * 3 >= 8 * 9 / 2: na <= 64
 // beam taps a producer keeps in flight
 template <typename T> constexpr int kGemmTaps() { return sizeof(T) == 4 ? 4 : 2; }
 
@@ -147,8 +153,8 @@ template <typename T, bool SHAPED, bool STAGE>
 __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs<T> &G,
                                             unsigned char *smem_raw, int t, int c, cx<T> *stage) {
     constexpr int kGemmThreads = gemm_threads<T, SHAPED>();
-    constexpr int kGemmConsumers = gemm_consumers<SHAPED>();
-    constexpr int kGemmMaxTp = gemm_max_tp<SHAPED>();
+    constexpr int kGemmConsumers = gemm_consumers<T, SHAPED>();
+    constexpr int kGemmMaxTp = gemm_max_tp<T, SHAPED>();
     const mbx_dims &d = A.d;
     const int NA = d.na.local_size;
     const int tid = threadIdx.x;
@@ -881,9 +887,9 @@ static int launch_forward_mx(const RimeArgs<float> &A, const int32_t *bmap, cons
     const int nR = Gp.rows / 16;
     Gp.ntp = nR * (nR + 1) / 2;
     Gp.st = f.stp;
-    constexpr int cw = gemm_consumers<false>() / 64;
+    constexpr int cw = gemm_consumers<float, false>() / 64;
     Gp.tpw = (Gp.ntp + cw - 1) / cw;
-    if (Gp.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "fused forward: %d antennas", na);
+    if (Gp.tpw > gemm_max_tp<float, false>()) return set_error(MBX_E_UNSUPPORTED, "fused forward: %d antennas", na);
     GemmArgs<float> Gs = Gp;
     Gs.vin = vin;
     Gs.st = f.mx.st;
@@ -953,9 +959,9 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
     if (np > 0) {
         G.st = gemm_st<T, false>(na, budget);
         const size_t lds = gemm_lds_bytes<T, false>(na, G.st);
-        constexpr int cw = gemm_consumers<false>() / 64;
+        constexpr int cw = gemm_consumers<T, false>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
-        if (G.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
+        if (G.tpw > gemm_max_tp<T, false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
         hipLaunchKernelGGL((k_point_gemm<T, false>), dim3(unsigned(nwg)), dim3(gemm_threads<T, false>()),
                            lds, stream, Ag, G);
         int r = hip_status(hipGetLastError(), "point gemm launch");
@@ -968,9 +974,9 @@ int launch_point_gemm(const RimeArgs<T> &A, const int32_t *bmap, cx<T> *dst, con
             if (!r && epilogue_wgs) *epilogue_wgs = nwg;
             return r;
         }
-        constexpr int cw = gemm_consumers<true>() / 64;
+        constexpr int cw = gemm_consumers<T, true>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
-        if (G.tpw > gemm_max_tp<true>()) return set_error(MBX_E_UNSUPPORTED, "shaped gemm: %d antennas", na);
+        if (G.tpw > gemm_max_tp<T, true>()) return set_error(MBX_E_UNSUPPORTED, "shaped gemm: %d antennas", na);
         G.st = gemm_st<T, true>(na, budget);
         const size_t lds = gemm_lds_bytes<T, true>(na, G.st);
         hipLaunchKernelGGL((k_point_gemm<T, true>), dim3(unsigned(nwg)), dim3(gemm_threads<T, true>()),
@@ -1008,9 +1014,9 @@ int launch_batched_mx(const RimeArgs<float> &A, const int32_t *bmap, cx<float> *
     if (any_point) {
         G.st = gemm_st<float, false>(na, budget);
         const size_t lds = gemm_lds_bytes<float, false>(na, G.st);
-        constexpr int cw = gemm_consumers<false>() / 64;
+        constexpr int cw = gemm_consumers<float, false>() / 64;
         G.tpw = (G.ntp + cw - 1) / cw;
-        if (G.tpw > gemm_max_tp<false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
+        if (G.tpw > gemm_max_tp<float, false>()) return set_error(MBX_E_UNSUPPORTED, "point gemm: %d antennas", na);
         hipLaunchKernelGGL((k_point_gemm<float, false, true>), grid, dim3(gemm_threads<float, false>()), lds,
                            stream, Ag, G);
         const int r = hip_status(hipGetLastError(), "batched point gemm launch");
