@@ -65,7 +65,7 @@ template <typename T, bool SHAPED> constexpr int gemm_max_tp() {
 }
 constexpr int kGemmMaxRowBlocks = 8;  // 4 * 9, 8 * 5 and 12 * 3 >= 8 * 9 / 2: na <= 64
 // beam taps a producer keeps in flight
-template <typename T> constexpr int kGemmTaps() { return sizeof(T) == 4 ? 4 : 2; }
+template <typename T, bool SHAPED> constexpr int kGemmTaps() { return sizeof(T) == 4 || !SHAPED ? 4 : 2; }
 
 static inline int gemm_rows(int na) { return ((2 * na + 15) / 16) * 16; }
 
@@ -323,7 +323,9 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                 // the tile loop); per source only the source's base is added.
                 const char *Xg = reinterpret_cast<const char *>(Xbase + buf * xstride);
                 const char *XY = reinterpret_cast<const char *>(XYbase + buf * xystride);
-                const int *kind = Kbase + buf * st;
+                // the tile's source kinds, one per lane (st <= 32), read per
+                // source with v_readlane instead of an LDS round trip
+                const int kinds = lane < nst ? Kbase[buf * st + lane] : 0;
                 for (int s = 0; s < nst; ++s) {
                     // the source's byte offsets, opaque to the compiler: it
                     // would otherwise step every per-tile-pair address of
@@ -334,7 +336,7 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                     const char *Xs = Xg + xo;
                     const char *XYs = XY + yo;
                     // the kind is the same for the whole wave (a uniform branch)
-                    const bool gauss = kind[s] == 1;
+                    const bool gauss = __builtin_amdgcn_readlane(kinds, s) == 1;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
@@ -429,7 +431,7 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                 if (a >= NA) { a -= NA; ++si; }
                 if (si >= nst) break;
                 const SrcRec<T> &r = P[si];
-                const J4<T> J = staged_jones<T, kGemmTaps<T>()>(A, Luvw, Lpa, Lant, Lch, c, a, r.l0,
+                const J4<T> J = staged_jones<T, kGemmTaps<T, SHAPED>()>(A, Luvw, Lpa, Lant, Lch, c, a, r.l0,
                                                                 r.m0, r.nm1, r.B);
                 Xb[si * rows + 2 * a] = XRow<T>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};
                 Xb[si * rows + 2 * a + 1] = XRow<T>{{J.v[2].re, J.v[3].re, J.v[2].im, J.v[3].im}};
@@ -763,7 +765,7 @@ __device__ __forceinline__ void shaped_phase(const RimeArgs<float> &A, const Gem
                 if (a >= NA) { a -= NA; ++si; }
                 if (si >= nst) break;
                 const SrcRec<float> &sr = P[si];
-                const J4<float> J = staged_jones<float, kGemmTaps<float>()>(A, Luvw, Lpa, Lant, Lch, c, a,
+                const J4<float> J = staged_jones<float, kGemmTaps<float, true>()>(A, Luvw, Lpa, Lant, Lch, c, a,
                                                                             sr.l0, sr.m0, sr.nm1, sr.B);
                 XRow<float> *r = Xb + 4 * (si * NA + a);
                 r[0] = XRow<float>{{J.v[0].re, J.v[1].re, J.v[0].im, J.v[1].im}};

@@ -116,16 +116,39 @@ __device__ __forceinline__ J4<T> staged_jones(const RimeArgs<T> &A, const double
                                               const double *Lch, int c, int a, double l0,
                                               double m0, double nm1, const J4<T> &B) {
     const double *an = Lant + 4 * a;
-    const J4<T> E = beam_jones_at<T, UT>(A, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2], an[3],
-                                         c, l0, m0);
     const double *x = Luvw + 3 * a;
-    T sn, cs;
-    PhaseCS<T>::eval(-A.two_pi_over_c * Lch[3] * (x[2] * nm1 + x[1] * m0 + x[0] * l0), &sn, &cs);
-    const cx<T> K = {cs, sn};
-    J4<T> KB;
+    if constexpr (UT == 4) {
+        // the four beam cells are loaded first and the phase evaluated
+        // while they are in flight
+        const BeamTapAddr<T> ta =
+            beam_taps<T>(A, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2], an[3], c, l0, m0);
+        BeamCell<T> e[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
-    return jmul(E, KB);
+        for (int tap = 0; tap < 4; ++tap) e[tap] = ta.cell(tap);
+        T sn, cs;
+        PhaseCS<T>::eval(-A.two_pi_over_c * Lch[3] * (x[2] * nm1 + x[1] * m0 + x[0] * l0), &sn, &cs);
+        const cx<T> K = {cs, sn};
+        J4<T> KB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
+        cx<T> acc[4];
+        T amp[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) { acc[p] = {T(0), T(0)}; amp[p] = T(0); }
+#pragma unroll
+        for (int tap = 0; tap < 4; ++tap) beam_accumulate(e[tap], tap, ta.ld, ta.md, acc, amp);
+        return jmul(beam_normalise(acc, amp), KB);
+    } else {
+        const J4<T> E = beam_jones_at<T, UT>(A, Lpa[2 * a], Lpa[2 * a + 1], an[0], an[1], an[2], an[3],
+                                             c, l0, m0);
+        T sn, cs;
+        PhaseCS<T>::eval(-A.two_pi_over_c * Lch[3] * (x[2] * nm1 + x[1] * m0 + x[0] * l0), &sn, &cs);
+        const cx<T> K = {cs, sn};
+        J4<T> KB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) KB.v[q] = cmul(K, B.v[q]);
+        return jmul(E, KB);
+    }
 }
 
 // Per-antenna pointing error and scaling of time t, channel c into LDS.

@@ -432,10 +432,23 @@ __device__ __forceinline__ void tile_phase0(const RimeArgs<T> &A, Tile<T> &L, in
 // given the antenna's rotation (sin, cos of the parallactic angle),
 // pointing error and scaling for this (time, channel c). Grid coordinates
 // in float64. UT of the four bilinear taps are loaded together.
-template <typename T, int UT = 1>
-__device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, double pc,
-                                               double pel, double pem, double asl, double asm_,
-                                               int c, double l0, double m0) {
+// The taps of one beam lookup: byte offsets in the channel's plane of the
+// (0, 0) cell and of the +1 neighbours (0 at the last row / column), and
+// the bilinear fractions.
+template <typename T> struct BeamTapAddr {
+    const char *plane;
+    uint32_t o00, dl, dm;
+    T ld, md;
+    __device__ __forceinline__ const BeamCell<T> &cell(int tap) const {
+        return *reinterpret_cast<const BeamCell<T> *>(plane + o00 + ((tap & 1) ? dl : 0u) +
+                                                      ((tap & 2) ? dm : 0u));
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ BeamTapAddr<T> beam_taps(const RimeArgs<T> &A, double ps, double pc,
+                                                    double pel, double pem, double asl, double asm_,
+                                                    int c, double l0, double m0) {
     const mbx_dims &d = A.d;
     double l = (l0 * pc - m0 * ps + pel) * asl;
     double m = (l0 * ps + m0 * pc + pem) * asm_;
@@ -444,15 +457,56 @@ __device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, 
     double gl0 = floor(vl);
     double vm = fmin(MH1, fmax(0.0, A.mscale * (m - A.lmin)));
     double gm0 = floor(vm);
-    const T ld = T(vl - gl0), md = T(vm - gm0);
+    BeamTapAddr<T> r;
+    r.ld = T(vl - gl0);
+    r.md = T(vm - gm0);
     const uint32_t MH = d.beam_mh.local_size;
-    // byte offsets of the four taps in the channel's plane (a plane is a
-    // few MB at most: 32-bit, 24-bit multiplies); the + 1 neighbours
-    // clamp to the last row / column (g1 = min(g0 + 1, N - 1))
+    // a plane is a few MB at most: 32-bit offsets, 24-bit multiplies
     const uint32_t cell = uint32_t(sizeof(BeamCell<T>));
-    const uint32_t o00 = uint32_t(__umul24(uint32_t(__umul24(uint32_t(gl0), MH)) + uint32_t(gm0), cell));
-    const uint32_t dl = gl0 < LW1 ? uint32_t(__umul24(MH, cell)) : 0u, dm = gm0 < MH1 ? cell : 0u;
-    const char *plane = reinterpret_cast<const char *>(A.beam + uint64_t(c) * d.beam_lw.local_size * MH);
+    r.o00 = uint32_t(__umul24(uint32_t(__umul24(uint32_t(gl0), MH)) + uint32_t(gm0), cell));
+    r.dl = gl0 < LW1 ? uint32_t(__umul24(MH, cell)) : 0u;
+    r.dm = gm0 < MH1 ? cell : 0u;
+    r.plane = reinterpret_cast<const char *>(A.beam + uint64_t(c) * d.beam_lw.local_size * MH);
+    return r;
+}
+
+// Weighted sums of the values and amplitudes of tap `tap` (taps in order 0..3).
+template <typename T>
+__device__ __forceinline__ void beam_accumulate(const BeamCell<T> &e, int tap, T ld, T md, cx<T> *acc, T *amp) {
+    const bool hl = tap & 1, hm = (tap >> 1) & 1;
+    const T w = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        acc[p].re = fma(w, e.v[3 * p], acc[p].re);
+        acc[p].im = fma(w, e.v[3 * p + 1], acc[p].im);
+        amp[p] = fma(w, e.v[3 * p + 2], amp[p]);
+    }
+}
+
+// E = value * amp / |value| per correlation.
+template <typename T>
+__device__ __forceinline__ J4<T> beam_normalise(const cx<T> *acc, const T *amp) {
+    J4<T> E;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        T n2 = acc[p].re * acc[p].re + acc[p].im * acc[p].im;
+        T g = (n2 > T(0) && isfinite(n2)) ? amp[p] * rsqrt_acc(n2) : amp[p];
+        E.v[p] = {acc[p].re * g, acc[p].im * g};
+    }
+    return E;
+}
+
+// Beam Jones from the prepared planes (RimeEBeam.py:233-343 /
+// CPUSolver.py:348-475 semantics: trilinear interpolation of the values
+// and, separately, of their amplitudes, then E = value * amp / |value|)
+// given the antenna's rotation (sin, cos of the parallactic angle),
+// pointing error and scaling for this (time, channel c). Grid coordinates
+// in float64. UT of the four bilinear taps are loaded together.
+template <typename T, int UT = 1>
+__device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, double pc,
+                                               double pel, double pem, double asl, double asm_,
+                                               int c, double l0, double m0) {
+    const BeamTapAddr<T> ta = beam_taps<T>(A, ps, pc, pel, pem, asl, asm_, c, l0, m0);
     cx<T> acc[4];
     T amp[4];
 #pragma unroll
@@ -462,25 +516,10 @@ __device__ __forceinline__ J4<T> beam_jones_at(const RimeArgs<T> &A, double ps, 
     for (int tap0 = 0; tap0 < 4; tap0 += UT)
 #pragma unroll
     for (int u = 0; u < UT; ++u) {
-        const int tap = tap0 + u;
-        const bool hl = tap & 1, hm = (tap >> 1) & 1;
-        const T w = (hl ? ld : T(1) - ld) * (hm ? md : T(1) - md);
-        const BeamCell<T> e = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + (hl ? dl : 0u) + (hm ? dm : 0u));
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            acc[p].re = fma(w, e.v[3 * p], acc[p].re);
-            acc[p].im = fma(w, e.v[3 * p + 1], acc[p].im);
-            amp[p] = fma(w, e.v[3 * p + 2], amp[p]);
-        }
+        const BeamCell<T> e = ta.cell(tap0 + u);
+        beam_accumulate(e, tap0 + u, ta.ld, ta.md, acc, amp);
     }
-    J4<T> E;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        T n2 = acc[p].re * acc[p].re + acc[p].im * acc[p].im;
-        T g = (n2 > T(0) && isfinite(n2)) ? amp[p] * rsqrt_acc(n2) : amp[p];
-        E.v[p] = {acc[p].re * g, acc[p].im * g};
-    }
-    return E;
+    return beam_normalise(acc, amp);
 }
 
 // Beam Jones of one (source, antenna, channel), reading the antenna's
