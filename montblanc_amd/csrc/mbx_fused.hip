@@ -511,6 +511,34 @@ static FwdPlan plan_forward(const mbx_dims &d, const mbx_pair_plan *pp) {
     return p;
 }
 
+// The epilogue alone, once the matrix-core passes have summed every
+// source into A.vin: M = G_p V G_q^H, flags, model / residual and chi^2
+// terms per visibility (vis_out), one thread per visibility of a time
+// step (grid y strides over time, x over baselines x channels), one chi^2
+// partial per workgroup in the grid's row-major order.
+template <typename T>
+__global__ void __launch_bounds__(kGenThreads) k_epilogue(RimeArgs<T> A) {
+    __shared__ double part[kGenThreads / 64];
+    const mbx_dims &d = A.d;
+    const int NA = d.na.local_size;
+    const uint64_t NBL = d.nbl.local_size, NC = d.nchan.local_size;
+    const uint32_t nb = uint32_t(dext(d.nbl)), nc = uint32_t(dext(d.nchan));
+    const uint32_t n = nb * nc, nt = uint32_t(dext(d.ntime));
+    const bool final_batch = d.nsrc.upper_extent >= d.nsrc.global_size;
+    double term = 0.0;
+    for (uint32_t t = blockIdx.y; t < nt; t += gridDim.y)
+    for (uint32_t i = blockIdx.x * kGenThreads + threadIdx.x; i < n; i += gridDim.x * kGenThreads) {
+        const uint32_t b = i / nc, c = i - b * nc;
+        const int p = min(max(A.a1[uint64_t(t) * NBL + b], 0), NA - 1);
+        const int q = min(max(A.a2[uint64_t(t) * NBL + b], 0), NA - 1);
+        const uint64_t vi = (uint64_t(t) * NBL + b) * NC + c;
+        const J4<T> V = jload(A.vin + vi * 4);
+        term += vis_out(A, int(t), int(b), int(c), p, q, V, final_batch);
+    }
+    wg_partial(term, part, A.partial + (uint64_t(blockIdx.y) * gridDim.x + blockIdx.x), threadIdx.x,
+               kGenThreads);
+}
+
 // Upper bound on workgroups of any forward kernel (one float64 partial
 // each): time x (channel tiles padded to kChanGroup) x baseline chunks
 // (at least 64 pair blocks or baselines per chunk), padded to 8 XCDs.
@@ -641,6 +669,27 @@ static int forward(const mbx_dims *d, const mbx_inputs *in, const mbx_outputs *o
         }
         A.vin = vbuf;
         A.s_begin = shaped ? dext(d->nsrc) : dext(d->npsrc);
+        if (A.s_begin >= dext(d->nsrc)) {
+            // every source summed: the epilogue alone, about four
+            // visibilities per thread
+            const uint64_t nbc = uint64_t(dext(d->nbl)) * dext(d->nchan);
+            const uint64_t gx = std::max<uint64_t>(1, std::min<uint64_t>((nbc + 4 * kGenThreads - 1) /
+                                                                          (4 * kGenThreads), 1024));
+            const uint64_t gy = std::min<uint64_t>(dext(d->ntime), 65535);
+            const uint64_t nwg = gx * gy;
+            if (nwg > max_partials(*d) || nbc >= (uint64_t(1) << 32))
+                return set_error(MBX_E_UNSUPPORTED, "forward epilogue: %llu workgroups",
+                                 (unsigned long long)nwg);
+            hipLaunchKernelGGL((k_epilogue<T>), dim3(unsigned(gx), unsigned(gy)),
+                               dim3(kGenThreads), 0, stream, A);
+            r = hip_status(hipGetLastError(), "forward epilogue launch");
+            if (r) return r;
+            if (final_batch) {
+                launch_chi2_final(partial, unsigned(nwg), scale, out->X2, stream);
+                r = hip_status(hipGetLastError(), "forward chi2 launch");
+            }
+            return r;
+        }
     }
     A.ct = p.ct; A.st = p.st; A.nbchunk = p.nbchunk; A.bchunk = p.bchunk; A.nctile = p.nctile;
     A.per_xcd = uint32_t(p.nwg / 8);
