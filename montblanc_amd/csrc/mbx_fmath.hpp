@@ -1,5 +1,5 @@
-// Float64 shape-factor arithmetic for the matrix-core passes: exp(x) for
-// x <= 0 and d^(-3/2) for d >= 1, in fewer vector instructions than the
+// Float64 shape-factor arithmetic for the matrix-core passes: exp(-x) for
+// x >= 0 and d^(-3/2) for d >= 1, in fewer vector instructions than the
 // general library routines (which handle every range and special value).
 #pragma once
 
@@ -34,14 +34,18 @@ __device__ __forceinline__ double exp_neg(double x, const double *tab) {
     const long long xb = __builtin_bit_cast(long long, x);
     const int xhi = min(int(xb >> 32), 0x40874C00);   // high word of 745.5
     x = __builtin_bit_cast(double, (static_cast<long long>(xhi) << 32) | static_cast<unsigned int>(xb));
-    const double k = rint(x * -kInvStep);
+    // k = round(-x 256 / ln 2) by the 1.5 2^52 shift: the sum's low word
+    // holds k (|k| < 2^19) as a two's complement integer
+    constexpr double kShift = 0x1.8p52;
+    const double kd = fma(x, -kInvStep, kShift);
+    const int ki = int(static_cast<unsigned int>(__builtin_bit_cast(unsigned long long, kd)));
+    const double k = kd - kShift;
     double r = fma(-k, kStepHi, -x);
     r = fma(-k, kStepLo, r);
     double p = fma(r, 1.0 / 24, 1.0 / 6);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    const int ki = int(k);
     return ldexp(p * tab[ki & (kExpTab - 1)], ki >> 8);   // ki = 256 m + j, m = floor(ki / 256)
 }
 
