@@ -102,6 +102,35 @@ def test_shaped_matrix_pass_applies_epilogue(residual, use_w):
     check_outputs(out, ref, a, np.float32, residual, use_w, 1.7)
 
 
+@pytest.mark.parametrize('residual,use_w,keep', [(False, False, True), (True, False, False),
+                                                 (False, True, False)])
+def test_float64_epilogue_kernel(residual, use_w, keep):
+    """Float64 from 48 antennas: the point and shaped matrix-core passes sum
+    every source, then k_epilogue (one thread per visibility) applies gains,
+    flags, residual mode, weights and the chi^2 terms, with baselines in
+    either antenna order and auto-correlations; with keep_ekb the pre-gain
+    sums stay in ekb_vis."""
+    from test_gpu_forward import check_outputs, run_forward
+    counts = (5, 6, 4)
+    a = synth.make_inputs(48, 2, 3, *counts, beam=(9, 8, 6), seed=81, flag_fraction=0.3,
+                          auto_correlations=True, observed='model')
+    a1, a2 = a['antenna1'].copy(), a['antenna2'].copy()
+    flip = np.zeros(a1.shape[1], bool)
+    flip[::3] = True
+    a['antenna1'] = np.where(flip, a2, a1).astype(a1.dtype)
+    a['antenna2'] = np.where(flip, a1, a2).astype(a2.dtype)
+    if use_w:
+        a['weight_vector'] = np.random.default_rng(4).uniform(0.5, 2.0, a['weight_vector'].shape)
+    a['sigma_sqrd'] = 1.7
+    ref = rime_np.rime(a, *counts, residual=residual, use_weight_vector=use_w, sigma_sqrd=1.7)
+    out = run_forward(a, counts, np.float64, residual=residual, use_weight_vector=use_w, keep_ekb=keep)
+    check_outputs(out, ref, a, np.float64, residual, use_w, 1.7)
+    if keep:
+        ekb = out['ekb_vis']
+        ekb = ekb.cpu().numpy() if hasattr(ekb, 'cpu') else np.asarray(ekb)
+        assert np.abs(ekb - ref['ekb_vis']).max() <= 1e-10 * np.abs(ref['ekb_vis']).max()
+
+
 @pytest.mark.parametrize('dtype,na', [(np.float32, 44), (np.float64, 48)])
 def test_shaped_matrix_pass_without_point_sources(dtype, na):
     """No point sources: the point pass is skipped and the shaped matrix
