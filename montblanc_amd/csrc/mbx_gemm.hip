@@ -337,6 +337,8 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                     const char *XYs = XY + yo;
                     // the kind is the same for the whole wave (a uniform branch)
                     const bool gauss = __builtin_amdgcn_readlane(kinds, s) == 1;
+                    // row antenna coordinates: the same for tile pairs of one row block
+                    typename Vec2<T>::type cp;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
@@ -347,8 +349,8 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                             const acc_t Q = Mfma<T>::mma(flip_sign(a2, sgn), bv, zero);
                             const typename Vec2<T>::type cq =
                                 *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offQ[j]);
-                            const typename Vec2<T>::type cp =
-                                *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offP[j]);
+                            if (j == 0 || tpR[j] != tpR[j - 1])
+                                cp = *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offP[j]);
                             const T dx = cq.x - cp.x, dy = cq.y - cp.y;
                             T fm;
                             if constexpr (sizeof(T) == 8) {
