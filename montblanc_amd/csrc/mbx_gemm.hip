@@ -337,20 +337,24 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                     const char *XYs = XY + yo;
                     // the kind is the same for the whole wave (a uniform branch)
                     const bool gauss = __builtin_amdgcn_readlane(kinds, s) == 1;
-                    // row antenna coordinates: the same for tile pairs of one row block
+                    // the A operands and row antenna coordinates: the same for
+                    // the tile pairs of one row block (consecutive in the
+                    // wave's row-major share), read once per row block
+                    T av, a2f;
                     typename Vec2<T>::type cp;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
+                            if (j == 0 || tpR[j] != tpR[j - 1]) {
+                                av = *reinterpret_cast<const T *>(Xs + offA[j]);
+                                a2f = flip_sign(*reinterpret_cast<const T *>(Xs + offA[j] + dA2), sgn);
+                                cp = *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offP[j]);
+                            }
                             const T bv = *reinterpret_cast<const T *>(Xs + offB[j]);
-                            const acc_t P =
-                                Mfma<T>::mma(*reinterpret_cast<const T *>(Xs + offA[j]), bv, zero);
-                            const T a2 = *reinterpret_cast<const T *>(Xs + offA[j] + dA2);
-                            const acc_t Q = Mfma<T>::mma(flip_sign(a2, sgn), bv, zero);
+                            const acc_t P = Mfma<T>::mma(av, bv, zero);
+                            const acc_t Q = Mfma<T>::mma(a2f, bv, zero);
                             const typename Vec2<T>::type cq =
                                 *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offQ[j]);
-                            if (j == 0 || tpR[j] != tpR[j - 1])
-                                cp = *reinterpret_cast<const typename Vec2<T>::type *>(XYs + offP[j]);
                             const T dx = cq.x - cp.x, dy = cq.y - cp.y;
                             T fm;
                             if constexpr (sizeof(T) == 8) {
