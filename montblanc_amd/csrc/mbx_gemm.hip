@@ -287,10 +287,15 @@ __device__ __forceinline__ void point_phase(const RimeArgs<T> &A, const GemmArgs
                     int xo = s0 * rows * int(sizeof(XRow<T>));
                     __asm__ volatile("" : "+s"(xo));
                     const char *Xs = Xb + xo;
+                    // float: A rows read once per row block of the wave's
+                    // tile pairs (consecutive in its row-major share); float64
+                    // reads them per tile pair (measured faster there)
+                    XRow<T> a;
 #pragma unroll
                     for (int j = 0; j < kGemmMaxTp; ++j) {
                         if (j < ntp_mine) {
-                            const XRow<T> a = *reinterpret_cast<const XRow<T> *>(Xs + offRa[j]);
+                            if (sizeof(T) == 8 || j == 0 || tpR[j] != tpR[j - 1])
+                                a = *reinterpret_cast<const XRow<T> *>(Xs + offRa[j]);
                             const XRow<T> b = *reinterpret_cast<const XRow<T> *>(Xs + offCb[j]);
                             if constexpr (kGauss3) {
 #pragma unroll
