@@ -164,6 +164,11 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
     const BeamCell<T> e10 = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + dl);
     const BeamCell<T> e01 = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + dm);
     const BeamCell<T> e11 = *reinterpret_cast<const BeamCell<T> *>(plane + o00 + dl + dm);
+    // ---- phase, evaluated while the four cells are in flight
+    const double phi = -A.two_pi_over_c * f * (xa[2] * nm1 + xa[1] * m0 + xa[0] * l0);
+    T sn, cs;
+    PhaseCS<T>::eval(phi, &sn, &cs);
+    const cx<T> K = {cs, sn};
     T Bv[12], dBl[12], dBm[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
@@ -205,11 +210,7 @@ __device__ __forceinline__ J4<T> item_forward(const RimeArgs<T> &A, const double
             dEm.v[x] = {dSm[x].re * Am[x] + S[x].re * dAm[x], dSm[x].im * Am[x] + S[x].im * dAm[x]};
         }
     }
-    // ---- phase and sqrt(B)
-    const double phi = -A.two_pi_over_c * f * (xa[2] * nm1 + xa[1] * m0 + xa[0] * l0);
-    T sn, cs;
-    PhaseCS<T>::eval(phi, &sn, &cs);
-    const cx<T> K = {cs, sn};
+    // ---- sqrt(B)
     const T rP = sb.rP, s0 = sb.s0, it0 = sb.it0;
     J4<T> B0s;                            // sqrt(B0), B0 = unscaled brightness
     B0s.v[0] = {I + Q + s0, T(0)}; B0s.v[1] = {U, V}; B0s.v[2] = {U, -V}; B0s.v[3] = {I - Q + s0, T(0)};
